@@ -1,0 +1,184 @@
+#!/usr/bin/env python
+"""Docking throughput on one MI355X: ligands/sec over the 1,024-ligand uneven batch.
+
+Workload (synthetic, seeded): receptor of 2,000 heavy atoms in a shell around
+the pocket, 30 A box, FP32 grid maps for 8 atom types at 0.375 A (81^3 points);
+1,024 ligands with 10-60 heavy atoms and 0-15 rotatable bonds; 128 MC chains
+per ligand x 40 MC steps, each step a BFGS of (25 + natom_max) / 3 iterations;
+then clustering at 1.0 A, refinement (5 rounds x 2 BFGS iterations with the
+penalty-slope ramp), filtering (energy_range 10, num_pose 10) and final
+scoring. One "step" = one such pass over the whole batch with the inputs
+already resident on the GPU.
+
+Prints one JSON line. ``roofline`` covers the dominant kernel (the persistent
+MC kernel): achieved = grid-gather bytes of one launch (energy evaluations x
+32 B x heavy atoms, counted by the kernel) / its average duration measured
+with HIP events on its stream. ``cpu_baseline`` times the CPU oracle's whole
+per-ligand pipeline (direct receptor sum, as the reference scores) on a
+bounded sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "uni-dock2_amd"))
+
+METRIC = "ligands/sec @1,024 uneven batch, learned vs plain, top score within 0.1 kcal/mol"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-lig", type=int, default=1024)
+    ap.add_argument("--nchain", type=int, default=128)
+    ap.add_argument("--mc-steps", type=int, default=40)
+    ap.add_argument("--mode", choices=["grid", "direct"], default="grid")
+    ap.add_argument("--cpu-sample", type=int, default=2, help="ligands timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(ligands, receptor, box, nchain, mc_steps, opt_steps, n_sample, threads):
+    """CPU oracle, whole DockTask pipeline per ligand, direct receptor sum, FP32 arithmetic."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    O.build()
+    sc = O.Scene(receptor, box, "direct")
+    secs = 0.0
+    evals = 0
+    for i, lig in enumerate(ligands[:n_sample]):
+        topo = O.Topo(lig)
+        start = O.PoseArr.initial(lig, dtype=__import__("numpy").float32)
+        _, _, dt, ev = O.dock_ligand(topo, sc, start, i, nchain, mc_steps, opt_steps, 5, 12345, 1.0, 10.0, 10,
+                                     True, nthreads=threads, sfx="f32")
+        secs += dt
+        evals += ev
+    return {
+        "value": round(n_sample / secs, 6),
+        "unit": "ligands/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"first {n_sample} ligands of the batch, full pipeline ({nchain} chains x {mc_steps} MC steps, "
+                   f"BFGS {opt_steps}, cluster, refine, score), direct receptor sum, FP32, {threads} threads, "
+                   f"{secs:.1f} s, {evals} energy evaluations"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(ligands are independent: see DESIGN.md for how the batch would shard)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from unidock_engine import _native as N
+    from unidock_engine import synthetic as S
+    from unidock_engine.engine import DockEngine, DockParams, heuristic_opt_steps
+
+    N.check(N.lib().ud2_set_device(0), "ud2_set_device")
+    t0 = time.time()
+    receptor = S.make_receptor()
+    box = S.docking_box()
+    synth = S.config1_ligands(args.n_lig)
+    ligands = [s.ligand for s in synth]
+    t_gen = time.time() - t0
+    print(f"# workload generated in {t_gen:.1f}s: {len(ligands)} ligands, "
+          f"{sum(l.natom for l in ligands)} atoms, {sum(l.npair for l in ligands)} intra pairs", file=sys.stderr,
+          flush=True)
+    mode = N.MODE_GRID if args.mode == "grid" else N.MODE_DIRECT
+    eng = DockEngine(receptor, box, mode=mode, spacing=S.GRID_SPACING)
+    ls = N.LigSet(ligands)
+    opt_steps = heuristic_opt_steps(max(l.natom for l in ligands))
+    prm = DockParams(nchain=args.nchain, mc_steps=args.mc_steps, opt_steps=opt_steps, refine_steps=5, num_pose=10,
+                     rmsd_limit=1.0, energy_range=10.0, seed=12345, randomize=True, mode=mode)
+    pb = N.PoseBuffers(ls, args.nchain)
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+
+    mc_times = []
+
+    for i in range(args.warmup):
+        eng.dock(ls, prm, poses=pb)
+        print(f"# warmup step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    mc_times.clear()
+    stats.zero_()
+    results = None
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        results, _ = eng.dock(ls, prm, poses=pb, mc_events=mc_times, mc_stats=stats)
+        print(f"# timed step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    ms_per_step = wall * 1000.0 / args.steps
+    value = len(ligands) * args.steps / wall
+
+    st = stats.cpu().numpy().astype(np.float64)
+    launches = len(mc_times)
+    mc_ms = np.asarray([a.elapsed_time(b) for a, b in mc_times])
+    # one ud2_mc call = one kernel launch per atoms-per-lane class; this batch has one class
+    bytes_per_launch = st[1] / max(1, launches)
+    achieved = bytes_per_launch / (float(mc_ms.mean()) / 1000.0) / 1e9 if launches else 0.0
+    top = [float(r.energies[0][0]) for r in results if r.energies]
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "ligands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded receptor shell and random torsion-tree ligands)",
+        "config": {
+            "workload": f"{len(ligands)}-ligand uneven batch (10-60 heavy atoms, 0-15 rotatable bonds), "
+                        f"{args.nchain} chains x {args.mc_steps} MC steps x BFGS {opt_steps}, cluster + refine + "
+                        f"score, {args.mode} inter term",
+            "method": "plain MC (learned proposal not enabled)",
+            "n_ligands": len(ligands),
+            "nchain": args.nchain,
+            "mc_steps": args.mc_steps,
+            "opt_steps": opt_steps,
+            "inter_mode": args.mode,
+            "mc_ms_per_launch": round(float(mc_ms.mean()), 3) if launches else None,
+            "energy_evals_per_step": int(st[0] / args.steps),
+            "mc_steps_per_sec": round(len(ligands) * args.nchain * args.mc_steps * args.steps / wall, 1),
+            "evals_per_sec": round(st[0] / wall, 1),
+            "median_top_affinity": round(float(np.median(top)), 4) if top else None,
+        },
+        "roofline": {
+            "kernel": "mc_kernel<1>",
+            "bound": "hbm",
+            "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+        },
+    }
+    if not args.no_cpu and args.cpu_sample > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(ligands, receptor, box, args.nchain, args.mc_steps, opt_steps,
+                                           args.cpu_sample, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

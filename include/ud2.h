@@ -1,0 +1,162 @@
+/*
+ * ud2.h: C ABI of the MI355X (gfx950) docking engine library, libud2.so.
+ *
+ * Plain C types only: host pointers are marked _h, device pointers _d (HIP
+ * device memory, e.g. allocated by the caller with hipMalloc or torch). Every
+ * function returns UD2_OK (0) or an error code; ud2_last_error() returns the
+ * calling thread's message for the last failure. Streams are hipStream_t
+ * passed as void* (NULL = default stream). No exception crosses the ABI.
+ *
+ * Reference citations use E/ = unidock/unidock_engine/ of Uni-Dock2.
+ */
+#ifndef UD2_H
+#define UD2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UD2_OK 0
+#define UD2_ERR_ARG 1      /* invalid argument or input topology */
+#define UD2_ERR_HIP 2      /* a HIP runtime call failed */
+#define UD2_ERR_NODEV 3    /* no GPU / device id out of range */
+#define UD2_ERR_LIMIT 4    /* input beyond a compiled limit (atoms, torsions, types) */
+#define UD2_ERR_STATE 5    /* object in the wrong state (e.g. grid mode without maps) */
+
+#define UD2_MODE_DIRECT 0  /* inter-molecular term as the direct receptor sum (reference semantics) */
+#define UD2_MODE_GRID 1    /* inter-molecular term from trilinear grid maps */
+
+#define UD2_MAX_ATOMS 256
+#define UD2_MAX_TORSIONS 57
+
+typedef struct ud2_scene ud2_scene;   /* receptor, docking box, grid maps (device) */
+typedef struct ud2_ligset ud2_ligset; /* a batch of ligand topologies (device) */
+
+/* Ligand batch in CSR form. Atom, torsion and pair indices inside one ligand
+ * are local (0-based within that ligand). Field meaning follows FlexTopo and
+ * FlexParamVina (E/src/model/model.h:95-108, E/src/score/vina.h:17-25). */
+typedef struct {
+    int32_t n_lig;
+    const int32_t* atom_off_h;   /* [n_lig+1] */
+    const int32_t* tors_off_h;   /* [n_lig+1] */
+    const int32_t* pair_off_h;   /* [n_lig+1] */
+    const int32_t* types_h;      /* [atoms] XS type (E/src/constants/constants.h:48-68) */
+    const float* xyz_h;          /* [atoms*3] input conformation */
+    const int32_t* axis_h;       /* [tors*2] (from, to) */
+    const float* dihe0_h;        /* [tors] input dihedrals, radians */
+    const int32_t* rot_off_h;    /* [tors+1] CSR into rot_atoms_h */
+    const int32_t* rot_atoms_h;  /* atoms rotated by each torsion, its "to" atom included */
+    const int32_t* range_off_h;  /* [tors+1] CSR into ranges_h, in (lo, hi) pairs */
+    const float* ranges_h;       /* allowed dihedral intervals (lo, hi), radians */
+    const int32_t* pairs_h;      /* [pairs*2] intra-molecular pairs (i, j) */
+} ud2_ligand_batch;
+
+/* Pose buffers for nchain poses per ligand of a ligand set (device memory).
+ * Layout (atom_off / tors_off are the batch's CSR offsets):
+ *   x    [(atom_off[l]*nchain + c*natom_l)*3 + 3*a + k]
+ *   dihe [tors_off[l]*nchain + c*ntors_l + t]
+ *   c, rv [(l*nchain + c)*3 + k]   centre, rotation vector of the last rotation increment
+ *   e    [l*nchain + c]            energy
+ * Pose id of (ligand l, chain c) = l*nchain + c. */
+typedef struct {
+    float* x_d;
+    float* c_d;
+    float* rv_d;
+    float* dihe_d;
+    float* e_d;
+    int32_t nchain;
+} ud2_poses;
+
+typedef struct {
+    int32_t mc_steps;        /* DockParam::mc_steps (E/src/model/model.h:156) */
+    int32_t opt_steps;       /* BFGS iterations per MC step; 0 = no local optimisation */
+    int32_t randomize;       /* randomise the input pose first (E/src/cuda/mc.cu:109-134) */
+    uint32_t seed;           /* key of each chain's Philox stream, with the ligand index */
+    int32_t mode;            /* UD2_MODE_DIRECT or UD2_MODE_GRID */
+    int32_t lig_index_base;  /* added to the ligand index in the random-stream key */
+    uint64_t* stats_d;       /* optional device counters, accumulated: [0] energy evaluations,
+                                [1] grid-gather bytes (evals * 32 * heavy atoms), [2] intra-pair
+                                evaluations, [3] chains done */
+} ud2_mc_params;
+
+/* ------------------------------------------------------------- library */
+const char* ud2_version(void);
+const char* ud2_last_error(void);
+int ud2_device_count(int32_t* n_h);
+int ud2_set_device(int32_t device_id);  /* cudaSetDevice in decide_memory_limit_mb, E/src/screening/core.cpp:39 */
+int ud2_synchronize(void);
+
+/* ------------------------------------------------------------- receptor
+ * Replaces the receptor half of DockTask::alloc_gpu (E/src/cuda/task.cu:304-402):
+ * heavy receptor atoms (hydrogens are dropped here, as the reference's parser
+ * does, E/src/format/rapidjson_parser.cpp:34) and the docking box. */
+int ud2_scene_create(const float* rec_xyz_h, const int32_t* rec_types_h, int32_t n_rec, const float box_lo_h[3],
+                     const float box_hi_h[3], ud2_scene** out);
+/* Grid maps over the box (no reference counterpart): map_t(p) = sum over heavy
+ * receptor atoms within 8 A of the uncurled pair energy Vina::eval_ef
+ * (E/src/score/vina.h:216-245) of a probe of type t. Points at box_lo +
+ * spacing*(i, j, k), n = round(size/spacing) + 1 per axis. */
+int ud2_scene_build_grids(ud2_scene* sc, float spacing, const int32_t* probe_types_h, int32_t n_types);
+/* Prebuilt maps, [n_types][nz][ny][nx] at origin + spacing*(i, j, k). */
+int ud2_scene_set_grids(ud2_scene* sc, const int32_t n_h[3], const float origin_h[3], float spacing,
+                        const int32_t* probe_types_h, int32_t n_types, const float* maps_h);
+int ud2_scene_grid_info(const ud2_scene* sc, int32_t n_h[3], float origin_h[3], float* spacing_h, int32_t* n_types_h);
+int ud2_scene_download_maps(const ud2_scene* sc, float* maps_h);
+int ud2_scene_destroy(ud2_scene* sc);
+
+/* ------------------------------------------------------------- ligands
+ * Replaces the ligand half of DockTask::alloc_gpu (E/src/cuda/task.cu:16-284). */
+int ud2_ligset_create(const ud2_ligand_batch* batch, ud2_ligset** out);
+int ud2_ligset_destroy(ud2_ligset* ls);
+/* Sizes of the pose buffers (floats) for nchain poses per ligand. */
+int ud2_ligset_pose_layout(const ud2_ligset* ls, int32_t nchain, int64_t* x_floats_h, int64_t* dihe_floats_h,
+                           int64_t* n_pose_h);
+/* Every pose set to the input conformation: centre = centroid, rot_vec = 0,
+ * energy 999 (E/src/cuda/task.cu:94-108). */
+int ud2_init_poses(const ud2_ligset* ls, const ud2_poses* P, void* stream);
+
+/* ------------------------------------------------------------- hot path
+ * E + gradient of pose vectors [centre(3), quaternion w,x,y,z (4), torsions]
+ * of ligand `lig`: the torsions are set on its input conformation in index
+ * order, the conformer is rotated by q about its centroid and moved to the
+ * centre, rot_vec = rotvec(q). Batched form of cal_e_grad_one_gpu
+ * (E/src/cuda/bfgs_wrapper.cu:63-98). e_d[n], g_d[n][6+ntors]. */
+int ud2_score_posevecs(const ud2_scene* sc, const ud2_ligset* ls, int32_t lig, int64_t n, const float* pv_d,
+                       float* e_d, float* g_d, int32_t mode, float v_cap, float slope, void* stream);
+/* E + gradient (cal_e_grad_tile, E/src/cuda/bfgs.cuh:397-410) of the selected
+ * poses (pose ids sel_d[n_sel]); g_d is [n_sel][64], components 0..5+ntors. */
+int ud2_energy_grad(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, int32_t mode, float v_cap, float slope, float* e_d, float* g_d, void* stream);
+/* One pose update x <- x (+) alpha * dir (apply_grad_update_pose_tile,
+ * E/src/cuda/bfgs.cuh:560-618); dir_d is [n_sel][64]. */
+int ud2_pose_update(const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
+                    const float* dir_d, float alpha, void* stream);
+/* One backtracking line search from H = I (line_search_one_gpu,
+ * E/src/cuda/bfgs_wrapper.cu:104-159), in place; alpha_d[n_sel], g_d[n_sel][64]. */
+int ud2_line_search(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, int32_t mode, float v_cap, float slope, float* alpha_d, float* g_d, void* stream);
+/* In-place BFGS of the selected poses (optimize_cu, E/src/cuda/optimize.cu:197-212). */
+int ud2_optimize(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                 int32_t n_sel, int32_t mode, int32_t max_steps, float v_cap, float slope, float* e_d, void* stream);
+/* Monte Carlo search, nchain chains per ligand, best pose per chain written to
+ * P (mc_cu, E/src/cuda/mc.cu:408-448). One persistent launch per atoms-per-lane
+ * class over a size-sorted work queue. */
+int ud2_mc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_mc_params* params,
+           void* stream);
+/* Greedy leader clustering per ligand by heavy-atom RMSD (cluster_cu,
+ * E/src/cuda/cluster.cu:74-135): leaders_d[n_lig][nchain] in selection order
+ * (chain indices), nleaders_d[n_lig]. */
+int ud2_cluster(const ud2_ligset* ls, const ud2_poses* P, float rmsd_limit, int32_t* leaders_d,
+                int32_t* nleaders_d, void* stream);
+/* Final re-score (score(), E/src/score/score.cpp:12-108): out3_d[n_sel][3] =
+ * intra (pairs once, no curl), inter (direct sum, no curl), box penalty. */
+int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, float* out3_d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UD2_H */

@@ -1,0 +1,375 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libud2_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the reported CPU baseline. The
+engine itself never imports this module.
+
+Ligand and receptor arguments are duck-typed: any object with the attributes
+of ``unidock_engine.topology.Ligand`` / ``Receptor`` works.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libud2_oracle.so")
+
+_i32p = C.POINTER(C.c_int32)
+_f32p = C.POINTER(C.c_float)
+
+
+class _Ligand(C.Structure):
+    _fields_ = [("natom", C.c_int32), ("ntors", C.c_int32), ("npair", C.c_int32), ("types", _i32p),
+                ("axis", _i32p), ("rot_start", _i32p), ("rot_atoms", _i32p), ("range_start", _i32p),
+                ("ranges", _f32p), ("pairs", _i32p)]
+
+
+class _Scene(C.Structure):
+    _fields_ = [("rec_natom", C.c_int32), ("rec_xyz", _f32p), ("rec_types", _i32p), ("grid_n", C.c_int32 * 3),
+                ("grid_origin", C.c_float * 3), ("grid_spacing", C.c_float), ("grid_maps", _f32p),
+                ("slot_of_type", C.c_int32 * 21), ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3),
+                ("mode", C.c_int32)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp = C.c_void_p
+    L.ud2o_eval_ef.restype = C.c_double
+    L.ud2o_eval_ef.argtypes = [C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.ud2o_rmsd_f32.restype = C.c_float
+    L.ud2o_cluster.restype = C.c_int
+    L.ud2o_filter.restype = C.c_int
+    for sfx, T in (("f64", C.c_double), ("f32", C.c_float)):
+        getattr(L, f"ud2o_energy_forces_{sfx}").restype = T
+        getattr(L, f"ud2o_energy_forces_{sfx}").argtypes = [vp, vp, vp, T, T, vp, vp]
+        getattr(L, f"ud2o_energy_grad_{sfx}").restype = T
+        getattr(L, f"ud2o_energy_grad_{sfx}").argtypes = [vp, vp, vp, vp, vp, vp, T, T, vp, vp]
+        getattr(L, f"ud2o_apply_torsion_{sfx}").argtypes = [vp, vp, vp, vp, vp, C.c_int, T]
+        getattr(L, f"ud2o_apply_update_{sfx}").argtypes = [vp, vp, vp, vp, vp, vp, T]
+        getattr(L, f"ud2o_pose_from_vec_{sfx}").argtypes = [vp] * 8
+        getattr(L, f"ud2o_score_posevecs_{sfx}").restype = C.c_double
+        getattr(L, f"ud2o_score_posevecs_{sfx}").argtypes = [vp, vp, vp, vp, C.c_int, vp, T, T, vp, vp, C.c_int]
+        getattr(L, f"ud2o_line_search_{sfx}").argtypes = [vp] * 8 + [T, T, T] + [vp] * 7
+        getattr(L, f"ud2o_bfgs_{sfx}").restype = C.c_int
+        getattr(L, f"ud2o_bfgs_{sfx}").argtypes = [vp] * 7 + [C.c_int, T, T]
+        getattr(L, f"ud2o_randomize_{sfx}").argtypes = [vp] * 6 + [C.c_uint32, C.c_int, C.c_int, C.c_int]
+        getattr(L, f"ud2o_mutate_{sfx}").argtypes = [vp] * 6 + [C.c_uint32, C.c_int, C.c_int, C.c_int]
+        getattr(L, f"ud2o_mc_{sfx}").restype = C.c_double
+        getattr(L, f"ud2o_mc_{sfx}").argtypes = ([vp] * 6 + [C.c_int] * 7 + [C.c_uint32] + [vp] * 6 + [C.c_int])
+        getattr(L, f"ud2o_final_score_{sfx}").argtypes = [vp, vp, vp, vp]
+        getattr(L, f"ud2o_dock_ligand_{sfx}").restype = C.c_int
+        getattr(L, f"ud2o_dock_ligand_{sfx}").argtypes = ([vp] * 6 + [C.c_int] * 6 + [C.c_uint32, C.c_float,
+                                                           C.c_float, C.c_int] + [vp] * 4 + [C.c_int])
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Topo:
+    """Keeps the contiguous arrays of one ligand alive and exposes the C struct."""
+
+    def __init__(self, lig):
+        self.types = np.ascontiguousarray(lig.types, dtype=np.int32)
+        self.axis = np.ascontiguousarray(lig.axis, dtype=np.int32).reshape(-1)
+        self.rot_start = np.ascontiguousarray(lig.rot_start, dtype=np.int32)
+        self.rot_atoms = np.ascontiguousarray(lig.rot_atoms, dtype=np.int32)
+        if self.rot_atoms.size == 0:
+            self.rot_atoms = np.zeros(1, dtype=np.int32)
+        self.range_start = np.ascontiguousarray(lig.range_start, dtype=np.int32)
+        self.ranges = np.ascontiguousarray(lig.ranges, dtype=np.float32).reshape(-1)
+        if self.ranges.size == 0:
+            self.ranges = np.zeros(2, dtype=np.float32)
+        self.pairs = np.ascontiguousarray(lig.pairs, dtype=np.int32).reshape(-1)
+        if self.pairs.size == 0:
+            self.pairs = np.zeros(2, dtype=np.int32)
+        self.x0 = np.ascontiguousarray(lig.coords, dtype=np.float32).reshape(-1)
+        self.dihe0 = np.ascontiguousarray(lig.dihedrals, dtype=np.float32)
+        if self.dihe0.size == 0:
+            self.dihe0 = np.zeros(1, dtype=np.float32)
+        self.natom, self.ntors, self.npair = int(lig.natom), int(lig.ntors), int(lig.npair)
+        self.s = _Ligand(self.natom, self.ntors, self.npair,
+                         self.types.ctypes.data_as(_i32p), self.axis.ctypes.data_as(_i32p),
+                         self.rot_start.ctypes.data_as(_i32p), self.rot_atoms.ctypes.data_as(_i32p),
+                         self.range_start.ctypes.data_as(_i32p), self.ranges.ctypes.data_as(_f32p),
+                         self.pairs.ctypes.data_as(_i32p))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+class Scene:
+    """Receptor (+ optional grid maps) and box. ``maps`` is [nslot, nz, ny, nx] float32."""
+
+    def __init__(self, receptor, box, mode: str = "direct", maps=None, grid_n=None, grid_origin=None,
+                 grid_spacing: float = 0.0, slot_of_type=None):
+        self.xyz = np.ascontiguousarray(receptor.xyz, dtype=np.float32).reshape(-1)
+        self.types = np.ascontiguousarray(receptor.types, dtype=np.int32)
+        if self.xyz.size == 0:
+            self.xyz = np.zeros(3, dtype=np.float32)
+            self.types = np.zeros(1, dtype=np.int32)
+        self.maps = None if maps is None else np.ascontiguousarray(maps, dtype=np.float32)
+        s = _Scene()
+        s.rec_natom = int(receptor.xyz.shape[0])
+        s.rec_xyz = self.xyz.ctypes.data_as(_f32p)
+        s.rec_types = self.types.ctypes.data_as(_i32p)
+        if self.maps is not None:
+            for d in range(3):
+                s.grid_n[d] = int(grid_n[d])
+                s.grid_origin[d] = float(grid_origin[d])
+            s.grid_spacing = float(grid_spacing)
+            s.grid_maps = self.maps.ctypes.data_as(_f32p)
+        sot = slot_of_type if slot_of_type is not None else [-1] * 21
+        for t in range(21):
+            s.slot_of_type[t] = int(sot[t])
+        for d in range(3):
+            s.box_lo[d] = float(box.lo[d])
+            s.box_hi[d] = float(box.hi[d])
+        s.mode = 1 if mode == "grid" else 0
+        self.s = s
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _dt(sfx):
+    return np.float64 if sfx == "f64" else np.float32
+
+
+# ------------------------------------------------------------------ helpers
+def eval_ef(d: float, t1: int, t2: int):
+    dd = C.c_double()
+    e = lib().ud2o_eval_ef(float(d), int(t1), int(t2), C.byref(dd))
+    return e, dd.value
+
+
+def eval_decomp(d: float, t1: int, t2: int) -> np.ndarray:
+    out = np.zeros(5, dtype=np.float64)
+    lib().ud2o_eval_decomp(C.c_double(d), int(t1), int(t2), _p(out))
+    return out
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().ud2o_philox4x32_10(_p(c), _p(k), _p(o))
+    return o
+
+
+def chain_blocks(seed: int, lig: int, chain: int, n: int) -> np.ndarray:
+    o = np.zeros((n, 4), dtype=np.uint32)
+    lib().ud2o_chain_blocks(C.c_uint32(seed), C.c_uint32(lig), C.c_uint32(chain), int(n), _p(o))
+    return o
+
+
+class PoseArr:
+    """Host pose (x[3n], c[3], rv[3], dihe[nt], energy) in a given precision."""
+
+    def __init__(self, x, c, rv, dihe, energy=999.0, dtype=np.float64):
+        self.x = np.array(x, dtype=dtype).reshape(-1)
+        self.c = np.array(c, dtype=dtype).reshape(3)
+        self.rv = np.array(rv, dtype=dtype).reshape(3)
+        self.dihe = np.array(dihe, dtype=dtype).reshape(-1)
+        if self.dihe.size == 0:
+            self.dihe = np.zeros(1, dtype=dtype)
+        self.energy = np.array([energy], dtype=dtype)
+
+    def copy(self):
+        p = PoseArr(self.x, self.c, self.rv, self.dihe, float(self.energy[0]), self.x.dtype)
+        return p
+
+    @staticmethod
+    def initial(lig, dtype=np.float64):
+        """The start pose the reference builds from the input (E/src/cuda/task.cu:94-108)."""
+        return PoseArr(lig.coords.reshape(-1), lig.centroid(), np.zeros(3), lig.dihedrals, 999.0, dtype)
+
+
+def energy_forces(topo: Topo, x, scene: Scene, v_cap=10.0, slope=1e6, sfx="f64"):
+    T = _dt(sfx)
+    xx = np.ascontiguousarray(x, dtype=T).reshape(-1)
+    f = np.zeros_like(xx)
+    terms = np.zeros(3, dtype=T)
+    e = getattr(lib(), f"ud2o_energy_forces_{sfx}")(C.byref(topo.s), _p(xx), scene.ref, v_cap, slope, _p(f),
+                                                     _p(terms))
+    return e, f.reshape(-1, 3), terms
+
+
+def energy_grad(topo: Topo, pose: PoseArr, scene: Scene, v_cap=10.0, slope=1e6, sfx="f64"):
+    g = np.zeros(6 + topo.ntors, dtype=_dt(sfx))
+    terms = np.zeros(3, dtype=_dt(sfx))
+    e = getattr(lib(), f"ud2o_energy_grad_{sfx}")(C.byref(topo.s), _p(pose.x), _p(pose.c), _p(pose.rv),
+                                                   _p(pose.dihe), scene.ref, v_cap, slope, _p(g), _p(terms))
+    return e, g, terms
+
+
+def apply_torsion(topo: Topo, pose: PoseArr, t: int, inc: float, sfx="f64"):
+    getattr(lib(), f"ud2o_apply_torsion_{sfx}")(C.byref(topo.s), _p(pose.x), _p(pose.c), _p(pose.rv),
+                                                 _p(pose.dihe), int(t), inc)
+
+
+def apply_update(topo: Topo, pose: PoseArr, g, alpha: float, sfx="f64"):
+    gg = np.ascontiguousarray(g, dtype=_dt(sfx))
+    getattr(lib(), f"ud2o_apply_update_{sfx}")(C.byref(topo.s), _p(pose.x), _p(pose.c), _p(pose.rv),
+                                                _p(pose.dihe), _p(gg), alpha)
+
+
+def pose_from_vec(topo: Topo, pv, sfx="f64") -> PoseArr:
+    T = _dt(sfx)
+    pvv = np.ascontiguousarray(pv, dtype=T)
+    p = PoseArr(np.zeros(3 * topo.natom), np.zeros(3), np.zeros(3), np.zeros(max(topo.ntors, 1)), dtype=T)
+    getattr(lib(), f"ud2o_pose_from_vec_{sfx}")(C.byref(topo.s), _p(topo.x0), _p(topo.dihe0), _p(pvv), _p(p.x),
+                                                 _p(p.c), _p(p.rv), _p(p.dihe))
+    return p
+
+
+def score_posevecs(topo: Topo, scene: Scene, pvs, v_cap=10.0, slope=1e6, nthreads=1, sfx="f64"):
+    """E and gradient of each pose vector; returns (e[n], g[n, 6+nt], seconds)."""
+    T = _dt(sfx)
+    pv = np.ascontiguousarray(pvs, dtype=T)
+    n = pv.shape[0]
+    e = np.zeros(n, dtype=T)
+    g = np.zeros((n, 6 + topo.ntors), dtype=T)
+    dt = getattr(lib(), f"ud2o_score_posevecs_{sfx}")(C.byref(topo.s), _p(topo.x0), _p(topo.dihe0), scene.ref,
+                                                       n, _p(pv), v_cap, slope, _p(e), _p(g), int(nthreads))
+    return e, g, dt
+
+
+def line_search(topo: Topo, scene: Scene, pose: PoseArr, g, p, e0, v_cap=10.0, slope=1e6, sfx="f64"):
+    T = _dt(sfx)
+    out = pose.copy()
+    gn = np.zeros(6 + topo.ntors, dtype=T)
+    e = np.zeros(1, dtype=T)
+    a = np.zeros(1, dtype=T)
+    gg = np.ascontiguousarray(g, dtype=T)
+    pp = np.ascontiguousarray(p, dtype=T)
+    getattr(lib(), f"ud2o_line_search_{sfx}")(C.byref(topo.s), scene.ref, _p(pose.x), _p(pose.c), _p(pose.rv),
+                                               _p(pose.dihe), _p(gg), _p(pp), e0, v_cap, slope, _p(out.x),
+                                               _p(out.c), _p(out.rv), _p(out.dihe), _p(gn), _p(e), _p(a))
+    return out, gn, float(e[0]), float(a[0])
+
+
+def bfgs(topo: Topo, scene: Scene, pose: PoseArr, max_steps: int, v_cap=10.0, slope=1e6, sfx="f64") -> int:
+    """In-place BFGS; returns the number of energy evaluations."""
+    return getattr(lib(), f"ud2o_bfgs_{sfx}")(C.byref(topo.s), scene.ref, _p(pose.x), _p(pose.c), _p(pose.rv),
+                                               _p(pose.dihe), _p(pose.energy), int(max_steps), v_cap, slope)
+
+
+def randomize(topo: Topo, scene: Scene, pose: PoseArr, seed: int, lig_index: int, chain: int, nchain: int,
+              sfx="f64"):
+    getattr(lib(), f"ud2o_randomize_{sfx}")(C.byref(topo.s), scene.ref, _p(pose.x), _p(pose.c), _p(pose.rv),
+                                             _p(pose.dihe), C.c_uint32(seed), lig_index, chain, nchain)
+
+
+def mutate(topo: Topo, scene: Scene, pose: PoseArr, seed: int, lig_index: int, chain: int, skip_draws: int = 0,
+           sfx="f64"):
+    getattr(lib(), f"ud2o_mutate_{sfx}")(C.byref(topo.s), scene.ref, _p(pose.x), _p(pose.c), _p(pose.rv),
+                                          _p(pose.dihe), C.c_uint32(seed), lig_index, chain, skip_draws)
+
+
+def mc(topo: Topo, scene: Scene, start: PoseArr, lig_index: int, nchain: int, mc_steps: int, opt_steps: int,
+       seed: int, randomize_flag: bool = True, chain_begin: int = 0, chain_count: int | None = None,
+       nthreads: int = 1, sfx="f64"):
+    """MC chains of one ligand. Returns dict of arrays (x, c, rv, dihe, e), evals and seconds."""
+    T = _dt(sfx)
+    cc = nchain - chain_begin if chain_count is None else chain_count
+    x = np.zeros((cc, 3 * topo.natom), dtype=T)
+    c = np.zeros((cc, 3), dtype=T)
+    rv = np.zeros((cc, 3), dtype=T)
+    dh = np.zeros((cc, max(topo.ntors, 1)), dtype=T)
+    e = np.zeros(cc, dtype=T)
+    ev = C.c_longlong(0)
+    s = start if start.x.dtype == T else PoseArr(start.x, start.c, start.rv, start.dihe, float(start.energy[0]), T)
+    dt = getattr(lib(), f"ud2o_mc_{sfx}")(C.byref(topo.s), scene.ref, _p(s.x), _p(s.c), _p(s.rv), _p(s.dihe),
+                                           lig_index, nchain, chain_begin, cc, mc_steps, opt_steps,
+                                           int(bool(randomize_flag)), C.c_uint32(seed), _p(x), _p(c), _p(rv),
+                                           _p(dh), _p(e), C.byref(ev), int(nthreads))
+    return dict(x=x, c=c, rv=rv, dihe=dh[:, :topo.ntors], e=e), ev.value, dt
+
+
+def final_score(topo: Topo, x, scene: Scene, sfx="f64") -> np.ndarray:
+    T = _dt(sfx)
+    xx = np.ascontiguousarray(x, dtype=T).reshape(-1)
+    out = np.zeros(3, dtype=T)
+    getattr(lib(), f"ud2o_final_score_{sfx}")(C.byref(topo.s), _p(xx), scene.ref, _p(out))
+    return out
+
+
+def dock_ligand(topo: Topo, scene: Scene, start: PoseArr, lig_index: int, nchain: int, mc_steps: int,
+                opt_steps: int, refine_steps: int, seed: int, rmsd_limit=1.0, energy_range=10.0, num_pose=10,
+                randomize_flag=True, nthreads=1, sfx="f64"):
+    """Whole per-ligand docking task; returns (coords[k, 3n], energies[k, 7], seconds, evals)."""
+    T = _dt(sfx)
+    ox = np.zeros((max(num_pose, 1), 3 * topo.natom), dtype=T)
+    oe = np.zeros((max(num_pose, 1), 7), dtype=T)
+    secs = C.c_double(0)
+    ev = C.c_longlong(0)
+    s = start if start.x.dtype == T else PoseArr(start.x, start.c, start.rv, start.dihe, float(start.energy[0]), T)
+    n = getattr(lib(), f"ud2o_dock_ligand_{sfx}")(C.byref(topo.s), scene.ref, _p(s.x), _p(s.c), _p(s.rv),
+                                                   _p(s.dihe), lig_index, nchain, mc_steps, opt_steps, refine_steps,
+                                                   int(bool(randomize_flag)), C.c_uint32(seed), rmsd_limit,
+                                                   energy_range, num_pose, _p(ox), _p(oe), C.byref(secs),
+                                                   C.byref(ev), int(nthreads))
+    return ox[:n], oe[:n], secs.value, ev.value
+
+
+def rmsd_f32(a, b, types) -> float:
+    aa = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    bb = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+    tt = np.ascontiguousarray(types, dtype=np.int32)
+    return float(lib().ud2o_rmsd_f32(_p(aa), _p(bb), _p(tt), int(tt.size)))
+
+
+def cluster(energies, coords, types, limit: float) -> np.ndarray:
+    e = np.ascontiguousarray(energies, dtype=np.float32)
+    x = np.ascontiguousarray(coords, dtype=np.float32)
+    t = np.ascontiguousarray(types, dtype=np.int32)
+    out = np.zeros(e.size, dtype=np.int32)
+    n = lib().ud2o_cluster(int(e.size), _p(e), _p(x), _p(t), int(t.size), C.c_float(limit), _p(out))
+    return out[:n]
+
+
+def grid_build(receptor, n, origin, spacing, probe_types, nthreads=1) -> np.ndarray:
+    xyz = np.ascontiguousarray(receptor.xyz, dtype=np.float32)
+    ty = np.ascontiguousarray(receptor.types, dtype=np.int32)
+    nn = np.asarray(n, dtype=np.int32)
+    org = np.asarray(origin, dtype=np.float32)
+    pt = np.asarray(probe_types, dtype=np.int32)
+    out = np.zeros((len(pt), nn[2], nn[1], nn[0]), dtype=np.float32)
+    lib().ud2o_grid_build(int(xyz.shape[0]), _p(xyz), _p(ty), _p(nn), _p(org), C.c_float(spacing), int(len(pt)),
+                          _p(pt), _p(out), int(nthreads))
+    return out
+
+
+def grid_points(receptor, probe_type: int, pts) -> np.ndarray:
+    xyz = np.ascontiguousarray(receptor.xyz, dtype=np.float32)
+    ty = np.ascontiguousarray(receptor.types, dtype=np.int32)
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(p.shape[0], dtype=np.float64)
+    lib().ud2o_grid_points(int(xyz.shape[0]), _p(xyz), _p(ty), int(probe_type), int(p.shape[0]), _p(p), _p(out))
+    return out

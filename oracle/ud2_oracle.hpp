@@ -1,0 +1,957 @@
+// CPU oracle for the unidock_engine docking hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file restates, on the host and in plain C++,
+// the algorithm the reference runs on the device, so that the HIP engine can
+// be checked against it. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it, and only as a checker or a reported baseline;
+// the product never links or calls it.
+//
+// Every function is templated on the arithmetic type T (double for the FP64
+// oracle, float for a build that follows the reference's Real=float
+// arithmetic, E/CMakeLists.txt:77-83). Constants are kept at the values the
+// reference's float build uses (radii, weights, PI, EPSILON), promoted to T.
+//
+// Citations use the abbreviation E/ = unidock/unidock_engine/ of the
+// reference repository.
+//
+// Deliberate deviations, all documented where they occur:
+//  * The random numbers come from this project's own Philox4x32-10 stream
+//    (Salmon et al., SC'11), not cuRAND; the sequence of draws follows the
+//    reference's use of them (mc.cu:18-256) one Philox block per draw.
+//  * The intra-molecular sum visits each unordered pair once and applies the
+//    pair force to both atoms. The reference visits the symmetric adjacency
+//    list twice and scales the energy by 0.5 (bfgs.cuh:305-353); the two are
+//    equal in exact arithmetic because eval_ef is symmetric in its types.
+//  * Clustering sorts energies with a stable tie-break on the pose index; the
+//    reference's std::sort is unstable (cluster.cu:99-102).
+//  * A receptor grid-map mode (trilinear interpolation of precomputed
+//    per-type maps) exists beside the reference's direct sum; it has no
+//    reference counterpart.
+#pragma once
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace ud2o {
+
+// ---------------------------------------------------------------- constants
+// XS atom types, E/src/constants/constants.h:48-68.
+enum : int {
+    XS_H = 0, XS_B = 1, XS_C_H = 2, XS_C_P = 3, XS_N_P = 4, XS_N_D = 5, XS_N_A = 6,
+    XS_N_DA = 7, XS_O_P = 8, XS_O_D = 9, XS_O_A = 10, XS_O_DA = 11, XS_F_H = 12,
+    XS_Si = 13, XS_P_P = 14, XS_S_P = 15, XS_Cl_H = 16, XS_Br_H = 17, XS_I_H = 18,
+    XS_At = 19, XS_Met_D = 20, XS_NTYPES = 21
+};
+
+// vdW radii, E/src/constants/constants.h:21-43 (float values of the Real=float build).
+static const float kVdwRadius[XS_NTYPES] = {
+    1.1f, 1.92f, 1.9f, 1.9f, 1.8f, 1.8f, 1.8f, 1.8f, 1.7f, 1.7f, 1.7f,
+    1.7f, 1.5f, 2.2f, 2.1f, 2.0f, 1.8f, 2.0f, 2.2f, 2.3f, 1.2f};
+
+// Vina weights and term shapes, E/src/score/vina.h:98-123.
+static const float kGauss1Offset = 0.0f, kGauss1Width = 0.5f, kWGauss1 = -0.035579f;
+static const float kGauss2Offset = 3.0f, kGauss2Width = 2.0f, kWGauss2 = -0.005156f;
+static const float kWRepulsion = 0.840245f;
+static const float kHydLow = 0.5f, kHydHigh = 1.5f, kWHydrophobic = -0.035069f;
+static const float kHbLow = -0.7f, kHbHigh = 0.0f, kWHbond = -0.587439f;
+static const float kWConfIndep = 0.05846f;
+static const float kR2Cutoff = 64.0f;                       // vina.h:130
+static const float kPi = 3.1415927f;                         // common.h:45 (Real(3.1415927))
+static const float kEps = FLT_EPSILON;                       // common.h:13 (epsilon of Real=float)
+static const float kPenaltySlope = 1e6f;                     // constants.h:16
+static const float kVCap = 10.0f;                            // constants.h:18
+static const float kVCapAuthentic = 1000.0f;                 // constants.h:19
+static const int kLineSearchSteps = 10;                      // constants.h:11
+static const float kLineSearchC0 = 1e-4f;                    // constants.h:12
+static const float kLineSearchMult = 0.5f;                   // constants.h:13
+static const float kMetropolisT = 1.2f;                      // mc.cu:366
+static const float kMutationAmplitude = 2.0f;                // mc.cu:148
+static const double kSuperFibPhi = 1.4142135623730951;       // sqrt(2), Alexa CVPR'22 (common.h:46)
+static const double kSuperFibPsi = 1.533751168755204288118041; // Alexa CVPR'22 (common.h:47)
+
+// --------------------------------------------------------------- structures
+// Ligand topology; the fields follow FlexTopo/FlexParamVina (E/src/model/model.h:95-108,
+// E/src/score/vina.h:17-25) with CSR offsets instead of [start, count] pairs.
+struct Ligand {
+    int natom = 0, ntors = 0, npair = 0;
+    const int32_t* types = nullptr;       // [natom] XS type
+    const int32_t* axis = nullptr;        // [ntors*2] (from, to)
+    const int32_t* rot_start = nullptr;   // [ntors+1] CSR into rot_atoms
+    const int32_t* rot_atoms = nullptr;   // rotated atoms of each torsion (incl. the "to" atom)
+    const int32_t* range_start = nullptr; // [ntors+1] CSR in units of (lo, hi) pairs
+    const float* ranges = nullptr;        // [2 * range_start[ntors]] radians
+    const int32_t* pairs = nullptr;       // [npair*2] intra pairs (i, j)
+};
+
+struct Box { float lo[3], hi[3]; };
+
+struct Receptor {
+    int natom = 0;
+    const float* xyz = nullptr;     // [natom*3]
+    const int32_t* types = nullptr; // [natom]
+};
+
+// Grid maps: maps[slot][iz][iy][ix], x fastest; point (ix,iy,iz) sits at
+// origin + spacing*(ix,iy,iz). slot_of_type maps an XS type to its map (-1 = none).
+struct Grid {
+    int n[3] = {0, 0, 0};
+    float origin[3] = {0, 0, 0};
+    float spacing = 0;
+    const float* maps = nullptr;
+    int32_t slot_of_type[XS_NTYPES];
+};
+
+enum InterMode : int { INTER_DIRECT = 0, INTER_GRID = 1 };
+
+struct Scene {
+    const Receptor* rec = nullptr;
+    const Grid* grid = nullptr;
+    Box box;
+    int mode = INTER_DIRECT;
+};
+
+template <class T>
+struct Pose {
+    std::vector<T> x;     // [natom*3]
+    T c[3] = {0, 0, 0};   // centre
+    T rv[3] = {0, 0, 0};  // rotation vector of the last applied rotation increment
+    std::vector<T> dihe;  // [ntors] radians
+    T energy = T(999);    // model.h:19
+};
+
+template <class T>
+struct Terms { T inter = 0, intra = 0, penalty = 0; };
+
+// ------------------------------------------------------------- scalar math
+template <class T> inline T sq(T v) { return v * v; }
+
+// normalize_angle, E/src/myutils/mymath.h:38-47.
+template <class T> inline T normalize_angle(T x) {
+    const T pi = T(kPi);
+    x = std::fmod(x, T(2) * pi);
+    if (x < -pi) x += T(2) * pi;
+    if (x > pi) x -= T(2) * pi;
+    return x;
+}
+
+// normalize_angle_2pi, mymath.h:24-30.
+template <class T> inline T normalize_angle_2pi(T x) {
+    x = std::fmod(x, T(2) * T(kPi));
+    if (x < 0) x += T(2) * T(kPi);
+    return x;
+}
+
+// clamp_by_ranges, mymath.h:133-158.
+template <class T> inline T clamp_by_ranges(T ang, const float* ranges, int nrange) {
+    for (int i = 0; i < nrange; i++)
+        if (ang >= T(ranges[2 * i]) && ang <= T(ranges[2 * i + 1])) return ang;
+    T closest = ang, min_dist = T(100);
+    for (int i = 0; i < nrange; i++) {
+        T d_lo = std::fabs(ang - T(ranges[2 * i]));
+        T d_hi = std::fabs(ang - T(ranges[2 * i + 1]));
+        if (d_lo < min_dist) { min_dist = d_lo; closest = T(ranges[2 * i]); }
+        if (d_hi < min_dist) { min_dist = d_hi; closest = T(ranges[2 * i + 1]); }
+    }
+    return closest;
+}
+
+// clamp_by_range, mymath.h:122-130.
+template <class T> inline T clamp_by_range(T v, T hi, T lo) {
+    if (v > hi) return hi;
+    if (v < lo) return lo;
+    return v;
+}
+
+// get_radian_in_ranges, mymath.h:103-110. The range index is chosen in float,
+// as the device does, so that oracle and engine pick the same range from the
+// same random bits.
+template <class T> inline T radian_in_ranges(const float* ranges, int nrange, float u0, float u1) {
+    float fi = std::ceil(u0 * float(nrange) - 1.0f);
+    int i = int(fi);
+    if (i < 0) i = 0;
+    if (i > nrange - 1) i = nrange - 1;
+    T from = T(ranges[2 * i]), to = T(ranges[2 * i + 1]);
+    return from + (to - from) * T(u1);
+}
+
+// gaussian, mymath.h:210-214.
+template <class T> inline T gaussian(T d, T offset, T width, T* f) {
+    T e = std::exp(-sq((d - offset) / width));
+    *f = e * T(2) * (offset - d) / width / width;
+    return e;
+}
+
+// slope_interp, vina.h:75-87.
+template <class T> inline T slope_interp(T lo, T hi, T x, T* f) {
+    if (x <= lo) { *f = 0; return 1; }
+    if (x >= hi) { *f = 0; return 0; }
+    *f = T(1) / (lo - hi);
+    return (hi - x) / (hi - lo);
+}
+
+// Type predicates, vina.h:49-65.
+inline bool is_hydrophobic(int t) {
+    return t == XS_C_H || t == XS_F_H || t == XS_Cl_H || t == XS_Br_H || t == XS_I_H;
+}
+inline bool is_acceptor(int t) { return t == XS_N_A || t == XS_N_DA || t == XS_O_A || t == XS_O_DA; }
+inline bool is_donor(int t) {
+    return t == XS_N_D || t == XS_N_DA || t == XS_O_D || t == XS_O_DA || t == XS_Met_D;
+}
+inline bool hbond_possible(int a, int b) {
+    return (is_acceptor(a) && is_donor(b)) || (is_donor(a) && is_acceptor(b));
+}
+
+// Vina::eval_ef, vina.h:216-245: energy and dE/dd of one atom pair at surface distance d.
+template <class T> inline T eval_ef(T d, int t1, int t2, T* dedd) {
+    T e = 0, f = 0;
+    *dedd = 0;
+    if (t1 == XS_H || t2 == XS_H) return e;
+    e += T(kWGauss1) * gaussian(d, T(kGauss1Offset), T(kGauss1Width), &f);
+    *dedd += T(kWGauss1) * f;
+    e += T(kWGauss2) * gaussian(d, T(kGauss2Offset), T(kGauss2Width), &f);
+    *dedd += T(kWGauss2) * f;
+    // vina_repulsion, vina.h:161-168
+    T rep = 0;
+    f = 0;
+    if (d < T(0)) { f = T(2) * d; rep = d * d; }
+    e += T(kWRepulsion) * rep;
+    *dedd += T(kWRepulsion) * f;
+    if (is_hydrophobic(t1) && is_hydrophobic(t2)) {
+        e += T(kWHydrophobic) * slope_interp(T(kHydLow), T(kHydHigh), d, &f);
+        *dedd += T(kWHydrophobic) * f;
+    }
+    if (hbond_possible(t1, t2)) {
+        e += T(kWHbond) * slope_interp(T(kHbLow), T(kHbHigh), d, &f);
+        *dedd += T(kWHbond) * f;
+    }
+    return e;
+}
+
+// Individual weighted terms (gauss1, gauss2, repulsion, hydrophobic, hbond), vina.h:256-274.
+template <class T> inline void eval_decomp(T d, int t1, int t2, T out[5]) {
+    out[0] = out[1] = out[2] = out[3] = out[4] = 0;
+    if (t1 == XS_H || t2 == XS_H) return;
+    T f = 0;
+    out[0] = T(kWGauss1) * gaussian(d, T(kGauss1Offset), T(kGauss1Width), &f);
+    out[1] = T(kWGauss2) * gaussian(d, T(kGauss2Offset), T(kGauss2Width), &f);
+    out[2] = T(kWRepulsion) * (d < T(0) ? d * d : T(0));
+    if (is_hydrophobic(t1) && is_hydrophobic(t2))
+        out[3] = T(kWHydrophobic) * slope_interp(T(kHydLow), T(kHydHigh), d, &f);
+    if (hbond_possible(t1, t2)) out[4] = T(kWHbond) * slope_interp(T(kHbLow), T(kHbHigh), d, &f);
+}
+
+// ----------------------------------------------------------- vector algebra
+template <class T> inline void cross(const T* a, const T* b, T* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <class T> inline T dot3(const T* a, const T* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <class T> inline T norm3(const T* a) { return std::sqrt(dot3(a, a)); }
+
+// rotvec_to_quaternion, E/src/geometry/quaternion.h:90-107. q = (w, x, y, z).
+template <class T> inline void rotvec_to_quat(const T* v, T* q) {
+    T th = norm3(v);
+    if (th > T(kEps)) {
+        T s = std::sin(th / 2);
+        q[0] = std::cos(th / 2);
+        q[1] = s * v[0] / th;
+        q[2] = s * v[1] / th;
+        q[3] = s * v[2] / th;
+    } else {
+        q[0] = 1; q[1] = q[2] = q[3] = 0;
+    }
+}
+
+// quaternion_to_rotvec, quaternion.h:115-130.
+template <class T> inline void quat_to_rotvec(const T* q, T* v) {
+    T th = T(2) * std::acos(q[0]);
+    T s = std::sin(th / 2);
+    if (s > T(kEps)) {
+        T k = th / s;
+        v[0] = q[1] * k; v[1] = q[2] * k; v[2] = q[3] * k;
+    } else {
+        v[0] = v[1] = v[2] = 0;
+    }
+}
+
+// quaternion_multiply_left (out = a * b), quaternion.h:46-52.
+template <class T> inline void quat_mul(const T* a, const T* b, T* o) {
+    T r0 = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    T r1 = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    T r2 = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    T r3 = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = r0; o[1] = r1; o[2] = r2; o[3] = r3;
+}
+
+// rotate_vec_by_quaternion (v <- q v q*), quaternion.h:185-196.
+template <class T> inline void quat_rotate(const T* q, T* v) {
+    T p[4] = {0, v[0], v[1], v[2]}, t[4], qc[4] = {q[0], -q[1], -q[2], -q[3]}, r[4];
+    quat_mul(q, p, t);
+    quat_mul(t, qc, r);
+    v[0] = r[1]; v[1] = r[2]; v[2] = r[3];
+}
+
+// rotate_vec_by_rodrigues, E/src/geometry/rotation.h:56-66.
+template <class T> inline void rodrigues(T* v, const T* axis, T theta) {
+    T c = std::cos(theta), s = std::sin(theta);
+    T m = (T(1) - c) * dot3(axis, v), t[3];
+    cross(axis, v, t);
+    v[0] = c * v[0] + m * axis[0] + s * t[0];
+    v[1] = c * v[1] + m * axis[1] + s * t[1];
+    v[2] = c * v[2] + m * axis[2] + s * t[2];
+}
+
+// cal_grad_of_rot_over_vec, rotation.h:80-144 (with rotvec_to_axis_angle(norm=true), :19-46,
+// and skew/skew_sq/outer products of E/src/myutils/matrix.h:20-42).
+template <class T> inline void drot_dv(const T* v, int k, T* out) {
+    T ang = norm3(v), th = 0, vb[3] = {0, 0, 0};
+    if (ang > T(kEps)) {
+        th = normalize_angle_2pi(ang);
+        if (th > T(kPi)) { th = T(2) * T(kPi) - th; ang = -ang; }
+        vb[0] = v[0] / ang; vb[1] = v[1] / ang; vb[2] = v[2] / ang;
+    }
+    T s = std::sin(th), c = std::cos(th);
+    T K[9] = {0, -vb[2], vb[1], vb[2], 0, -vb[0], -vb[1], vb[0], 0};
+    T K2[9] = {-vb[1] * vb[1] - vb[2] * vb[2], vb[0] * vb[1], vb[0] * vb[2],
+               vb[0] * vb[1], -vb[0] * vb[0] - vb[2] * vb[2], vb[1] * vb[2],
+               vb[0] * vb[2], vb[1] * vb[2], -vb[0] * vb[0] - vb[1] * vb[1]};
+    T e[3] = {0, 0, 0};
+    e[k] = 1;
+    for (int j = 0; j < 9; j++) out[j] = c * vb[k] * K[j] + s * vb[k] * K2[j];
+    T sdt, omc;
+    if (th > T(kEps)) { sdt = s / th; omc = (T(1) - c) / th; }
+    else { T t2 = th * th; sdt = T(1) - t2 / T(6); omc = th / T(2) - th * t2 / T(24); }
+    T vs[3] = {e[0] - vb[k] * vb[0], e[1] - vb[k] * vb[1], e[2] - vb[k] * vb[2]};
+    T S[9] = {0, -vs[2], vs[1], vs[2], 0, -vs[0], -vs[1], vs[0], 0};
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) {
+            T mc = e[r] * vb[cc] + vb[r] * e[cc] - T(2) * vb[k] * vb[r] * vb[cc];
+            out[r * 3 + cc] += sdt * S[r * 3 + cc] + omc * mc;
+        }
+}
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): multipliers 0xD2511F53/0xCD9E8D57, Weyl key
+// increments 0x9E3779B9/0xBB67AE85, ten rounds.
+inline void philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+        uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Uniform in (0, 1] from the top 24 bits (as curand_uniform's range); exact in float.
+inline float u01(uint32_t u) { return float((u >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+// The random stream of one MC chain: key (seed, ligand index), counter
+// (chain index, draw index). Each draw consumes one Philox block.
+struct ChainRng {
+    uint32_t key[2];
+    uint32_t chain;
+    uint32_t draw = 0;
+    ChainRng(uint32_t seed, uint32_t lig, uint32_t chain_) : chain(chain_) { key[0] = seed; key[1] = lig; }
+    void block_at(uint32_t d, uint32_t out[4]) const {
+        uint32_t ctr[4] = {chain, d, 0x55443244u, 0u};
+        philox4x32_10(ctr, key, out);
+    }
+    void next(uint32_t out[4]) { block_at(draw++, out); }
+};
+
+// ----------------------------------------------------------- pose kinematics
+// apply_grad_update_dihe_tile, E/src/cuda/bfgs.cuh:443-549: one torsion step with
+// angle normalisation, range clamping, Rodrigues rotation about (from -> to)
+// pivoting at "to", then the centroid of ALL atoms is recomputed.
+template <class T> void apply_torsion(const Ligand& L, Pose<T>& p, int t, T inc_raw) {
+    T inc = normalize_angle(inc_raw);
+    int r0 = L.range_start[t], nr = L.range_start[t + 1] - r0;
+    T nw = clamp_by_ranges(normalize_angle(p.dihe[t] + inc), L.ranges + 2 * r0, nr);
+    inc = nw - p.dihe[t];
+    p.dihe[t] = nw;
+    int from = L.axis[2 * t], to = L.axis[2 * t + 1];
+    T xt[3] = {p.x[3 * to], p.x[3 * to + 1], p.x[3 * to + 2]};
+    T ax[3] = {xt[0] - p.x[3 * from], xt[1] - p.x[3 * from + 1], xt[2] - p.x[3 * from + 2]};
+    T l = norm3(ax);
+    if (l <= T(kEps)) { ax[0] = 1; ax[1] = 0; ax[2] = 0; }
+    else { ax[0] /= l; ax[1] /= l; ax[2] /= l; }
+    for (int k = L.rot_start[t]; k < L.rot_start[t + 1]; k++) {
+        int a = L.rot_atoms[k];
+        T v[3] = {p.x[3 * a] - xt[0], p.x[3 * a + 1] - xt[1], p.x[3 * a + 2] - xt[2]};
+        rodrigues(v, ax, inc);
+        p.x[3 * a] = v[0] + xt[0]; p.x[3 * a + 1] = v[1] + xt[1]; p.x[3 * a + 2] = v[2] + xt[2];
+    }
+    T s[3] = {0, 0, 0};
+    for (int a = 0; a < L.natom; a++) { s[0] += p.x[3 * a]; s[1] += p.x[3 * a + 1]; s[2] += p.x[3 * a + 2]; }
+    p.c[0] = s[0] / T(L.natom); p.c[1] = s[1] / T(L.natom); p.c[2] = s[2] / T(L.natom);
+}
+
+// apply_grad_update_pose_tile, bfgs.cuh:560-618: torsions in index order, then the
+// rigid step: centre += alpha*g_c, coordinates rotated about the OLD centre by
+// q(alpha*g_rot), rot_vec := alpha*g_rot. (clamp_to_center is the identity, mymath.h:112-119.)
+// g layout: [0..2] centre, [3..5] orientation, [6..6+ntors) torsions.
+template <class T> void apply_update(const Ligand& L, Pose<T>& p, const T* g, T alpha) {
+    for (int t = 0; t < L.ntors; t++) apply_torsion(L, p, t, g[6 + t] * alpha);
+    T c_old[3] = {p.c[0], p.c[1], p.c[2]};
+    p.c[0] = c_old[0] + alpha * g[0];
+    p.c[1] = c_old[1] + alpha * g[1];
+    p.c[2] = c_old[2] + alpha * g[2];
+    T v[3] = {g[3] * alpha, g[4] * alpha, g[5] * alpha}, q[4];
+    rotvec_to_quat(v, q);
+    p.rv[0] = v[0]; p.rv[1] = v[1]; p.rv[2] = v[2];
+    for (int a = 0; a < L.natom; a++) {
+        T w[3] = {p.x[3 * a] - c_old[0], p.x[3 * a + 1] - c_old[1], p.x[3 * a + 2] - c_old[2]};
+        quat_rotate(q, w);
+        p.x[3 * a] = w[0] + p.c[0]; p.x[3 * a + 1] = w[1] + p.c[1]; p.x[3 * a + 2] = w[2] + p.c[2];
+    }
+}
+
+// ----------------------------------------------------------------- scoring
+// Trilinear value and gradient of one map at position x. Positions outside the
+// map are clamped to it per axis, with zero derivative along a clamped axis.
+template <class T> T grid_eval(const Grid& G, int slot, const T* x, T* grad) {
+    const float* m = G.maps + size_t(slot) * size_t(G.n[0]) * G.n[1] * G.n[2];
+    int i0[3];
+    T fr[3];
+    bool clamped[3];
+    for (int d = 0; d < 3; d++) {
+        T gpos = (x[d] - T(G.origin[d])) / T(G.spacing);
+        clamped[d] = false;
+        T top = T(G.n[d] - 1);
+        if (gpos < 0) { gpos = 0; clamped[d] = true; }
+        if (gpos > top) { gpos = top; clamped[d] = true; }
+        int i = int(std::floor(gpos));
+        if (i > G.n[d] - 2) i = G.n[d] - 2;
+        i0[d] = i;
+        fr[d] = gpos - T(i);
+    }
+    auto at = [&](int dx, int dy, int dz) -> T {
+        return T(m[(size_t(i0[2] + dz) * G.n[1] + (i0[1] + dy)) * G.n[0] + (i0[0] + dx)]);
+    };
+    T c000 = at(0, 0, 0), c100 = at(1, 0, 0), c010 = at(0, 1, 0), c110 = at(1, 1, 0);
+    T c001 = at(0, 0, 1), c101 = at(1, 0, 1), c011 = at(0, 1, 1), c111 = at(1, 1, 1);
+    T fx = fr[0], fy = fr[1], fz = fr[2], gx = 1 - fx, gy = 1 - fy, gz = 1 - fz;
+    T v00 = c000 * gx + c100 * fx, v10 = c010 * gx + c110 * fx;
+    T v01 = c001 * gx + c101 * fx, v11 = c011 * gx + c111 * fx;
+    T v0 = v00 * gy + v10 * fy, v1 = v01 * gy + v11 * fy;
+    T val = v0 * gz + v1 * fz;
+    T dx = ((c100 - c000) * gy + (c110 - c010) * fy) * gz + ((c101 - c001) * gy + (c111 - c011) * fy) * fz;
+    T dy = (v10 - v00) * gz + (v11 - v01) * fz;
+    T dz = v1 - v0;
+    T inv = T(1) / T(G.spacing);
+    grad[0] = clamped[0] ? T(0) : dx * inv;
+    grad[1] = clamped[1] ? T(0) : dy * inv;
+    grad[2] = clamped[2] ? T(0) : dz * inv;
+    return val;
+}
+
+// One grid-map point: sum over heavy receptor atoms within the cutoff of the
+// uncurled pair energy for a probe of the given type (no reference counterpart).
+template <class T> T grid_point_value(const Receptor& R, int probe_type, const T* p) {
+    T s = 0, dd;
+    for (int j = 0; j < R.natom; j++) {
+        int tj = R.types[j];
+        if (tj == XS_H) continue;
+        T r[3] = {T(R.xyz[3 * j]) - p[0], T(R.xyz[3 * j + 1]) - p[1], T(R.xyz[3 * j + 2]) - p[2]};
+        T rr = dot3(r, r);
+        if (rr < T(kR2Cutoff)) {
+            T d = std::sqrt(rr);
+            s += eval_ef(d - (T(kVdwRadius[probe_type]) + T(kVdwRadius[tj])), probe_type, tj, &dd);
+        }
+    }
+    return s;
+}
+
+// cal_box_penalty, bfgs.cuh:196-218.
+template <class T> T box_penalty(const T* x, const Box& b, T* f, T slope) {
+    T pen = 0;
+    for (int d = 0; d < 3; d++) {
+        if (x[d] < T(b.lo[d])) { pen += T(b.lo[d]) - x[d]; f[d] += -slope; }
+        if (x[d] > T(b.hi[d])) { pen += x[d] - T(b.hi[d]); f[d] += slope; }
+    }
+    return pen * slope;
+}
+
+// cal_e_f_tile, bfgs.cuh:221-394: total energy under the curl cap v_cap, and
+// f = dE/dx per atom. Inter: direct sum (:242-290) or grid map; per-atom curl
+// (:276-283). Penalty on heavy atoms (:293-300). Intra with per-pair curl
+// (:305-353), each pair once (see the header). Bias terms are not modelled
+// (the reference's default bias is "no", E/src/screening/core.h:13).
+template <class T>
+T energy_forces(const Ligand& L, const T* x, const Scene& S, T v_cap, T slope, T* f, Terms<T>* terms = nullptr) {
+    std::fill(f, f + 3 * L.natom, T(0));
+    T e_inter = 0, e_pen = 0, e_intra = 0, dd;
+    for (int i = 0; i < L.natom; i++) {
+        int ti = L.types[i];
+        if (ti == XS_H) continue;
+        T ea = 0, fa[3] = {0, 0, 0};
+        if (S.mode == INTER_DIRECT) {
+            const Receptor& R = *S.rec;
+            for (int j = 0; j < R.natom; j++) {
+                int tj = R.types[j];
+                if (tj == XS_H) continue;
+                T r[3] = {T(R.xyz[3 * j]) - x[3 * i], T(R.xyz[3 * j + 1]) - x[3 * i + 1],
+                          T(R.xyz[3 * j + 2]) - x[3 * i + 2]};
+                T rr = dot3(r, r);
+                if (rr < T(kR2Cutoff)) {
+                    T d = std::sqrt(rr);
+                    ea += eval_ef(d - (T(kVdwRadius[ti]) + T(kVdwRadius[tj])), ti, tj, &dd);
+                    if (d < T(kEps)) d = T(kEps);
+                    T fdr = dd / d;
+                    fa[0] -= fdr * r[0]; fa[1] -= fdr * r[1]; fa[2] -= fdr * r[2];
+                }
+            }
+        } else {
+            int slot = S.grid->slot_of_type[ti];
+            ea = grid_eval(*S.grid, slot, x + 3 * i, fa);
+        }
+        if (ea > 0) {
+            T s = v_cap / (v_cap + ea);
+            ea *= s;
+            s *= s;
+            fa[0] *= s; fa[1] *= s; fa[2] *= s;
+        }
+        e_inter += ea;
+        f[3 * i] += fa[0]; f[3 * i + 1] += fa[1]; f[3 * i + 2] += fa[2];
+    }
+    for (int i = 0; i < L.natom; i++)
+        if (L.types[i] != XS_H) e_pen += box_penalty(x + 3 * i, S.box, f + 3 * i, slope);
+    for (int k = 0; k < L.npair; k++) {
+        int i = L.pairs[2 * k], j = L.pairs[2 * k + 1];
+        T r[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+        T rr = dot3(r, r);
+        if (rr < T(kR2Cutoff)) {
+            T d = std::sqrt(rr);
+            int ti = L.types[i], tj = L.types[j];
+            T ep = eval_ef(d - (T(kVdwRadius[ti]) + T(kVdwRadius[tj])), ti, tj, &dd);
+            if (d < T(kEps)) d = T(kEps);
+            T fdr = dd / d;
+            T fp[3] = {-fdr * r[0], -fdr * r[1], -fdr * r[2]};
+            if (ep > 0) {
+                T s = v_cap / (v_cap + ep);
+                ep *= s;
+                s *= s;
+                fp[0] *= s; fp[1] *= s; fp[2] *= s;
+            }
+            e_intra += ep;
+            f[3 * i] += fp[0]; f[3 * i + 1] += fp[1]; f[3 * i + 2] += fp[2];
+            f[3 * j] -= fp[0]; f[3 * j + 1] -= fp[1]; f[3 * j + 2] -= fp[2];
+        }
+    }
+    if (terms) { terms->inter = e_inter; terms->intra = e_intra; terms->penalty = e_pen; }
+    return e_inter + e_pen + e_intra;
+}
+
+// cal_grad_tile, bfgs.cuh:69-193: torsion torques projected on the unit axis,
+// centre gradient = sum of heavy-atom gradients, orientation gradient =
+// <sum f (x) xbar, dR/dv(rot_vec)> with xbar = R(rot_vec)^-1 (x - c).
+template <class T> void pose_gradient(const Ligand& L, const Pose<T>& p, const T* f, T* g) {
+    for (int t = 0; t < L.ntors; t++) {
+        int from = L.axis[2 * t], to = L.axis[2 * t + 1];
+        T tq[3] = {0, 0, 0};
+        for (int k = L.rot_start[t]; k < L.rot_start[t + 1]; k++) {
+            int a = L.rot_atoms[k];
+            if (L.types[a] == XS_H) continue;
+            T lv[3] = {p.x[3 * a] - p.x[3 * to], p.x[3 * a + 1] - p.x[3 * to + 1], p.x[3 * a + 2] - p.x[3 * to + 2]};
+            T c3[3];
+            cross(lv, f + 3 * a, c3);
+            tq[0] += c3[0]; tq[1] += c3[1]; tq[2] += c3[2];
+        }
+        T ax[3] = {p.x[3 * to] - p.x[3 * from], p.x[3 * to + 1] - p.x[3 * from + 1], p.x[3 * to + 2] - p.x[3 * from + 2]};
+        g[6 + t] = dot3(tq, ax) / norm3(ax);
+    }
+    T q[4];
+    rotvec_to_quat(p.rv, q);
+    q[1] = -q[1]; q[2] = -q[2]; q[3] = -q[3];
+    T gc[3] = {0, 0, 0}, mat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < L.natom; a++) {
+        if (L.types[a] == XS_H) continue;
+        T xb[3] = {p.x[3 * a] - p.c[0], p.x[3 * a + 1] - p.c[1], p.x[3 * a + 2] - p.c[2]};
+        quat_rotate(q, xb);
+        const T* fa = f + 3 * a;
+        gc[0] += fa[0]; gc[1] += fa[1]; gc[2] += fa[2];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) mat[r * 3 + c] += fa[r] * xb[c];
+    }
+    g[0] = gc[0]; g[1] = gc[1]; g[2] = gc[2];
+    for (int k = 0; k < 3; k++) {
+        T dR[9];
+        drot_dv(p.rv, k, dR);
+        T s = 0;
+        for (int j = 0; j < 9; j++) s += mat[j] * dR[j];
+        g[3 + k] = s;
+    }
+}
+
+// cal_e_grad_tile, bfgs.cuh:397-410.
+template <class T>
+T energy_grad(const Ligand& L, const Pose<T>& p, const Scene& S, T v_cap, T slope, T* g,
+              std::vector<T>& fbuf, Terms<T>* terms = nullptr) {
+    fbuf.resize(3 * L.natom);
+    T e = energy_forces(L, p.x.data(), S, v_cap, slope, fbuf.data(), terms);
+    pose_gradient(L, p, fbuf.data(), g);
+    return e;
+}
+
+// ---------------------------------------------------------- line search/BFGS
+template <class T> inline T gdot(const T* a, const T* b, int n) {
+    T s = 0;
+    for (int i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+// line_search_tile, bfgs.cuh:623-670: backtracking Armijo, alpha = 1, x0.5, at
+// most 10 trials. If no trial passes, the last trial's pose/energy are kept and
+// alpha is returned after one more halving.
+template <class T>
+void line_search(const Ligand& L, const Scene& S, const Pose<T>& x, const T* g, const T* p, T e0,
+                 T v_cap, T slope, Pose<T>& x_new, T* g_new, T* e_out, T* alpha_out, std::vector<T>& fbuf,
+                 int* n_evals = nullptr) {
+    int dim = 6 + L.ntors;
+    T pg = gdot(p, g, dim);
+    T alpha = 1, e_new = 0;
+    for (int trial = 0; trial < kLineSearchSteps; trial++) {
+        x_new = x;
+        apply_update(L, x_new, p, alpha);
+        e_new = energy_grad(L, x_new, S, v_cap, slope, g_new, fbuf);
+        if (n_evals) (*n_evals)++;
+        if (e_new - e0 < T(kLineSearchC0) * alpha * pg) break;
+        alpha *= T(kLineSearchMult);
+    }
+    *e_out = e_new;
+    *alpha_out = alpha;
+}
+
+// bfgs_tile, bfgs.cuh:729-829 (+ minus_mat_vec_product_tile :413-428,
+// bfgs_update_hessian_tile :682-712): inverse-Hessian BFGS from H = I, with the
+// convergence test on the previous gradient, the step-0 diagonal rescale, and
+// the restore of the start pose when the energy went up.
+template <class T>
+void bfgs(const Ligand& L, const Scene& S, Pose<T>& x, int max_steps, T v_cap, T slope,
+          std::vector<T>& fbuf, int* n_evals = nullptr) {
+    const int dim = 6 + L.ntors;
+    std::vector<T> g(dim), g_new(dim), g_ori(dim), p(dim), y(dim), mhy(dim), H(size_t(dim) * dim);
+    T E_ori = energy_grad(L, x, S, v_cap, slope, g.data(), fbuf);
+    if (n_evals) (*n_evals)++;
+    T E = E_ori, E1 = 0, alpha = 0;
+    std::fill(H.begin(), H.end(), T(0));
+    for (int i = 0; i < dim; i++) H[size_t(i) * dim + i] = 1;
+    g_new = g;
+    g_ori = g;
+    Pose<T> x_new = x, x_ori = x;
+    for (int step = 0; step < max_steps; step++) {
+        for (int i = 0; i < dim; i++) {
+            T s = 0;
+            for (int j = 0; j < dim; j++) s += H[size_t(i) * dim + j] * g[j];
+            p[i] = -s;
+        }
+        line_search(L, S, x, g.data(), p.data(), E, v_cap, slope, x_new, g_new.data(), &E1, &alpha, fbuf, n_evals);
+        for (int i = 0; i < dim; i++) y[i] = g_new[i] - g[i];
+        E = E1;
+        x = x_new;
+        T gg = gdot(g.data(), g.data(), dim);
+        if (std::sqrt(gg) < T(1e-5f)) break;
+        g = g_new;
+        if (step == 0) {
+            T yy = gdot(y.data(), y.data(), dim);
+            if (std::fabs(yy) > T(kEps)) {
+                T yp0 = gdot(y.data(), p.data(), dim);
+                for (int i = 0; i < dim; i++) H[size_t(i) * dim + i] = alpha * yp0 / yy;
+            }
+        }
+        T yp = gdot(y.data(), p.data(), dim);
+        if (alpha * yp < T(kEps)) continue;
+        for (int i = 0; i < dim; i++) {
+            T s = 0;
+            for (int j = 0; j < dim; j++) s += H[size_t(i) * dim + j] * y[j];
+            mhy[i] = -s;
+        }
+        T yhy = -gdot(y.data(), mhy.data(), dim);
+        T r = T(1) / (alpha * yp);
+        for (int i = 0; i < dim; i++)
+            for (int j = i; j < dim; j++) {
+                T u = alpha * r * (mhy[i] * p[j] + mhy[j] * p[i]) + alpha * alpha * (r * r * yhy + r) * p[i] * p[j];
+                H[size_t(i) * dim + j] += u;
+                if (j != i) H[size_t(j) * dim + i] += u;
+            }
+    }
+    if (E > E_ori) {
+        E = E_ori;
+        x = x_ori;
+    }
+    x.energy = E;
+}
+
+// ---------------------------------------------------------- MC / randomise
+// gyration_radius, E/src/cuda/common.cuh:106-122 (heavy atoms about the pose centre).
+template <class T> T gyration_radius(const Ligand& L, const Pose<T>& p) {
+    T s = 0;
+    int n = 0;
+    for (int a = 0; a < L.natom; a++) {
+        if (L.types[a] == XS_H) continue;
+        T d[3] = {p.x[3 * a] - p.c[0], p.x[3 * a + 1] - p.c[1], p.x[3 * a + 2] - p.c[2]};
+        s += dot3(d, d);
+        n++;
+    }
+    T r = n > 0 ? std::sqrt(s / T(n)) : T(0);
+    return r > 0 ? r : T(1);
+}
+
+// Super-Fibonacci orientation of sample idx out of n (Alexa, CVPR'22), as a
+// rotation vector; mc.cu:71-80.
+template <class T> void super_fibonacci_rotvec(int idx, int n, T* v) {
+    T s = T(idx) + T(0.5);
+    T r = std::sqrt(s / T(n)), R = std::sqrt(T(1) - s / T(n));
+    T a = T(2) * T(kPi) * s / T(kSuperFibPhi), b = T(2) * T(kPi) * s / T(kSuperFibPsi);
+    T q[4] = {r * std::sin(a), r * std::cos(a), R * std::sin(b), R * std::cos(b)};
+    quat_to_rotvec(q, v);
+}
+
+// randomize_pose_tile, mc.cu:18-104 (the non-constraint, non-align branch):
+// centre drawn in the box, super-Fibonacci orientation by chain index, torsions
+// drawn in a random allowed range; applied through apply_update with alpha = 1.
+// Draws: one block for the centre, then one block per torsion.
+template <class T> void randomize(const Ligand& L, const Box& box, Pose<T>& p, int idx, int n, ChainRng& rng) {
+    std::vector<T> g(6 + L.ntors, T(0));
+    uint32_t b[4];
+    rng.next(b);
+    for (int d = 0; d < 3; d++) g[d] = (T(box.lo[d]) + (T(box.hi[d]) - T(box.lo[d])) * T(u01(b[d]))) - p.c[d];
+    super_fibonacci_rotvec<T>(idx, n, &g[3]);
+    for (int t = 0; t < L.ntors; t++) {
+        rng.next(b);
+        int r0 = L.range_start[t], nr = L.range_start[t + 1] - r0;
+        int j = int(u01(b[0]) * float(nr));
+        if (j > nr - 1) j = nr - 1;
+        T lo = T(L.ranges[2 * (r0 + j)]), hi = T(L.ranges[2 * (r0 + j) + 1]);
+        g[6 + t] = (lo + (hi - lo) * T(u01(b[1]))) - p.dihe[t];
+    }
+    apply_update(L, p, g.data(), T(1));
+}
+
+// gen_4_rand_in_sphere, E/src/cuda/common.cuh:90-103: rejection sampling in the
+// unit ball; the acceptance test is done in float with no contraction.
+inline void rand_in_ball(ChainRng& rng, float out[3]) {
+    for (;;) {
+        uint32_t b[4];
+        rng.next(b);
+        float a0 = (u01(b[0]) - 0.5f) * 2.0f, a1 = (u01(b[1]) - 0.5f) * 2.0f, a2 = (u01(b[2]) - 0.5f) * 2.0f;
+        float r2 = a0 * a0;
+        r2 = r2 + a1 * a1;
+        r2 = r2 + a2 * a2;
+        if (r2 < 1.0f) { out[0] = a0; out[1] = a1; out[2] = a2; return; }
+    }
+}
+
+// mutate_pose_tile, mc.cu:146-256: one of translation, rigid rotation, or one
+// torsion set to a value drawn in its allowed ranges.
+template <class T> void mutate(const Ligand& L, const Box& box, Pose<T>& p, ChainRng& rng) {
+    uint32_t b[4];
+    rng.next(b);
+    int which = int(b[0] % uint32_t(2 + L.ntors));
+    if (which == 0) {
+        float r[3];
+        rand_in_ball(rng, r);
+        T amp = T(kMutationAmplitude), d[3];
+        for (int k = 0; k < 3; k++) {
+            d[k] = clamp_by_range(amp * T(r[k]) + p.c[k], T(box.hi[k]), T(box.lo[k])) - p.c[k];
+            p.c[k] = p.c[k] + d[k];
+        }
+        for (int a = 0; a < L.natom; a++)
+            for (int k = 0; k < 3; k++) p.x[3 * a + k] = p.x[3 * a + k] + d[k];
+    } else if (which == 1) {
+        float r[3];
+        rand_in_ball(rng, r);
+        T a = gyration_radius(L, p);
+        T v[3] = {T(kMutationAmplitude) / a * T(r[0]), T(kMutationAmplitude) / a * T(r[1]),
+                  T(kMutationAmplitude) / a * T(r[2])};
+        T q[4];
+        rotvec_to_quat(v, q);
+        p.rv[0] = v[0]; p.rv[1] = v[1]; p.rv[2] = v[2];
+        for (int at = 0; at < L.natom; at++) {
+            T w[3] = {p.x[3 * at] - p.c[0], p.x[3 * at + 1] - p.c[1], p.x[3 * at + 2] - p.c[2]};
+            quat_rotate(q, w);
+            p.x[3 * at] = w[0] + p.c[0]; p.x[3 * at + 1] = w[1] + p.c[1]; p.x[3 * at + 2] = w[2] + p.c[2];
+        }
+    } else {
+        int t = which - 2;
+        rng.next(b);
+        int r0 = L.range_start[t], nr = L.range_start[t + 1] - r0;
+        T target = radian_in_ranges<T>(L.ranges + 2 * r0, nr, u01(b[0]), u01(b[1]));
+        apply_torsion(L, p, t, target - p.dihe[t]);
+    }
+}
+
+// metropolis_accept, E/src/search/mc.h:12-19.
+template <class T> inline bool metropolis(T e_old, T e_new, T temp, T u) {
+    if (e_new < e_old) return true;
+    return u < std::exp((e_old - e_new) / temp);
+}
+
+struct McParams {
+    int mc_steps = 20;
+    int opt_steps = 0;
+    int randomize = 1;
+    uint32_t seed = 12345;
+};
+
+// One chain of mc_kernel, mc.cu:275-398 (with the randomize prologue of
+// randomize_pose, mc.cu:109-134): returns the best pose of the chain.
+template <class T>
+Pose<T> mc_chain(const Ligand& L, const Scene& S, const Pose<T>& start, int lig_index, int chain, int nchain,
+                 const McParams& P, std::vector<T>& fbuf, long long* n_evals = nullptr) {
+    ChainRng rng(P.seed, uint32_t(lig_index), uint32_t(chain));
+    Pose<T> out = start;
+    if (P.randomize) randomize(L, S.box, out, chain, nchain, rng);
+    Pose<T> acc = out, cand;
+    int evals = 0;
+    std::vector<T> g(6 + L.ntors);
+    if (P.mc_steps == 0) {
+        fbuf.resize(3 * L.natom);
+        acc.energy = energy_forces(L, acc.x.data(), S, T(kVCap), T(kPenaltySlope), fbuf.data());
+        evals++;
+        out = acc;
+    } else {
+        T best_e = T(1e9);
+        for (int step = 0; step < P.mc_steps; step++) {
+            cand = acc;
+            mutate(L, S.box, cand, rng);
+            if (P.opt_steps == 0) {
+                cand.energy = energy_grad(L, cand, S, T(kVCap), T(kPenaltySlope), g.data(), fbuf);
+                evals++;
+            } else {
+                bfgs(L, S, cand, P.opt_steps, T(kVCap), T(kPenaltySlope), fbuf, &evals);
+            }
+            uint32_t b[4];
+            rng.next(b);
+            bool accepted = metropolis(acc.energy, cand.energy, T(kMetropolisT), T(u01(b[0])));
+            if (step == 0 || accepted) {
+                acc = cand;
+                if (acc.energy < best_e) {
+                    if (P.opt_steps > 0) bfgs(L, S, acc, P.opt_steps, T(kVCapAuthentic), T(kPenaltySlope), fbuf, &evals);
+                    if (acc.energy < best_e) {
+                        out = acc;
+                        best_e = acc.energy;
+                    }
+                }
+            }
+        }
+    }
+    if (n_evals) *n_evals += evals;
+    return out;
+}
+
+// ----------------------------------------------------------- clustering
+// cal_rmsd_pair_warp, E/src/cuda/cluster.cu:16-37: heavy-atom RMSD with no fit,
+// computed in float, atoms in index order, x/y/z in order, no contraction.
+inline float rmsd_f32(const float* a, const float* b, const int32_t* types, int natom) {
+    float s = 0.0f;
+    int n = 0;
+    for (int i = 0; i < natom; i++) {
+        if (types[i] == XS_H) continue;
+        float d0 = a[3 * i] - b[3 * i];
+        float t0 = d0 * d0;
+        s = s + t0;
+        float d1 = a[3 * i + 1] - b[3 * i + 1];
+        float t1 = d1 * d1;
+        s = s + t1;
+        float d2 = a[3 * i + 2] - b[3 * i + 2];
+        float t2 = d2 * d2;
+        s = s + t2;
+        n++;
+    }
+    return std::sqrt(s / float(n));
+}
+
+// cluster_cu's greedy leader selection, cluster.cu:93-129: poses in ascending
+// energy (ties by index), a pose is a new leader when its RMSD to every
+// existing leader is >= limit. Returns the leaders in selection order.
+inline std::vector<int> cluster_leaders(int npose, const float* energies, const float* coords /*[npose][natom*3]*/,
+                                        const int32_t* types, int natom, float limit) {
+    // NaN energies rank last, as +inf; ties keep index order
+    auto key = [&](int i) { float e = energies[i]; return e != e ? INFINITY : e; };
+    std::vector<int> order(npose);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+    std::vector<int> leaders;
+    for (int idx : order) {
+        bool novel = true;
+        for (int l : leaders) {
+            int a = std::min(idx, l), b = std::max(idx, l);
+            float r = rmsd_f32(coords + size_t(a) * natom * 3, coords + size_t(b) * natom * 3, types, natom);
+            if (r < limit) { novel = false; break; }
+        }
+        if (novel) leaders.push_back(idx);
+    }
+    return leaders;
+}
+
+// run_filter, E/src/task/DockTask.cpp:188-220: keep the clustered poses within
+// energy_range of the best, at most num_pose of them, in ascending energy.
+inline std::vector<int> filter_poses(const std::vector<int>& clustered, const float* energies, float energy_range,
+                                     int num_pose) {
+    std::vector<int> idx(clustered.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return energies[clustered[a]] < energies[clustered[b]]; });
+    std::vector<int> out;
+    if (idx.empty()) return out;
+    float emin = energies[clustered[idx[0]]];
+    for (int j : idx) {
+        if (energies[clustered[j]] > emin + energy_range) break;
+        out.push_back(clustered[j]);
+        if (int(out.size()) >= num_pose) break;
+    }
+    return out;
+}
+
+// score(), E/src/score/score.cpp:12-108: final re-score with no curl, intra
+// pairs once, direct inter sum, penalty at the fixed slope. out = [intra, inter, penalty].
+template <class T> void final_score(const Ligand& L, const T* x, const Receptor& R, const Box& box, T out[3]) {
+    T e_intra = 0, e_inter = 0, e_pen = 0, dd;
+    for (int k = 0; k < L.npair; k++) {
+        int i = L.pairs[2 * k], j = L.pairs[2 * k + 1];
+        T r[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+        T rr = dot3(r, r);
+        if (rr < T(kR2Cutoff)) {
+            T d = std::sqrt(rr);
+            int ti = L.types[i], tj = L.types[j];
+            e_intra += eval_ef(d - (T(kVdwRadius[ti]) + T(kVdwRadius[tj])), ti, tj, &dd);
+        }
+    }
+    for (int i = 0; i < L.natom; i++) {
+        int ti = L.types[i];
+        if (ti == XS_H) continue;
+        for (int j = 0; j < R.natom; j++) {
+            int tj = R.types[j];
+            if (tj == XS_H) continue;
+            T r[3] = {T(R.xyz[3 * j]) - x[3 * i], T(R.xyz[3 * j + 1]) - x[3 * i + 1], T(R.xyz[3 * j + 2]) - x[3 * i + 2]};
+            T rr = dot3(r, r);
+            if (rr < T(kR2Cutoff)) {
+                T d = std::sqrt(rr);
+                e_inter += eval_ef(d - (T(kVdwRadius[ti]) + T(kVdwRadius[tj])), ti, tj, &dd);
+            }
+        }
+    }
+    T ftmp[3] = {0, 0, 0};
+    for (int i = 0; i < L.natom; i++)
+        if (L.types[i] != XS_H) e_pen += box_penalty(x + 3 * i, box, ftmp, T(kPenaltySlope));
+    out[0] = e_intra; out[1] = e_inter; out[2] = e_pen;
+}
+
+// vina_conf_indep, vina.h:202-204.
+template <class T> inline T conf_indep(T e, int nrot) { return e / (T(1) + T(kWConfIndep) * T(nrot)); }
+
+}  // namespace ud2o

@@ -1,0 +1,382 @@
+// C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY; see ud2_oracle.hpp).
+//
+// Loaded by oracle/oracle.py through ctypes. Each numeric entry point exists in
+// an FP64 (_f64) and an FP32 (_f32) flavour; pose buffers are in the flavour's
+// type, topology and receptor buffers are float/int32 as the engine stores them.
+#include <atomic>
+#include <chrono>
+#include <thread>
+
+#include "ud2_oracle.hpp"
+
+using namespace ud2o;
+
+extern "C" {
+
+typedef struct {
+    int32_t natom, ntors, npair;
+    const int32_t* types;
+    const int32_t* axis;
+    const int32_t* rot_start;
+    const int32_t* rot_atoms;
+    const int32_t* range_start;
+    const float* ranges;
+    const int32_t* pairs;
+} ud2o_ligand;
+
+typedef struct {
+    int32_t rec_natom;
+    const float* rec_xyz;
+    const int32_t* rec_types;
+    int32_t grid_n[3];
+    float grid_origin[3];
+    float grid_spacing;
+    const float* grid_maps;
+    int32_t slot_of_type[21];
+    float box_lo[3], box_hi[3];
+    int32_t mode;  // 0 direct, 1 grid
+} ud2o_scene;
+
+}  // extern "C"
+
+namespace {
+
+Ligand to_lig(const ud2o_ligand* l) {
+    Ligand L;
+    L.natom = l->natom; L.ntors = l->ntors; L.npair = l->npair;
+    L.types = l->types; L.axis = l->axis; L.rot_start = l->rot_start; L.rot_atoms = l->rot_atoms;
+    L.range_start = l->range_start; L.ranges = l->ranges; L.pairs = l->pairs;
+    return L;
+}
+
+struct SceneHolder {
+    Receptor rec;
+    Grid grid;
+    Scene scene;
+    explicit SceneHolder(const ud2o_scene* s) {
+        rec.natom = s->rec_natom; rec.xyz = s->rec_xyz; rec.types = s->rec_types;
+        for (int d = 0; d < 3; d++) { grid.n[d] = s->grid_n[d]; grid.origin[d] = s->grid_origin[d]; }
+        grid.spacing = s->grid_spacing;
+        grid.maps = s->grid_maps;
+        for (int t = 0; t < 21; t++) grid.slot_of_type[t] = s->slot_of_type[t];
+        scene.rec = &rec;
+        scene.grid = &grid;
+        for (int d = 0; d < 3; d++) { scene.box.lo[d] = s->box_lo[d]; scene.box.hi[d] = s->box_hi[d]; }
+        scene.mode = s->mode;
+    }
+};
+
+template <class T>
+Pose<T> load_pose(const Ligand& L, const T* x, const T* c, const T* rv, const T* dihe, T energy) {
+    Pose<T> p;
+    p.x.assign(x, x + 3 * L.natom);
+    for (int d = 0; d < 3; d++) { p.c[d] = c[d]; p.rv[d] = rv[d]; }
+    p.dihe.assign(dihe, dihe + L.ntors);
+    p.energy = energy;
+    return p;
+}
+
+template <class T>
+void store_pose(const Ligand& L, const Pose<T>& p, T* x, T* c, T* rv, T* dihe, T* energy) {
+    std::copy(p.x.begin(), p.x.end(), x);
+    for (int d = 0; d < 3; d++) { c[d] = p.c[d]; rv[d] = p.rv[d]; }
+    std::copy(p.dihe.begin(), p.dihe.end(), dihe);
+    if (energy) *energy = p.energy;
+}
+
+// Pose vector [centre(3), quaternion w,x,y,z (4), torsions(ntors)] applied to the
+// reference conformation (x0, dihe0): torsions set in index order through the
+// torsion step, then the conformer rotated about its centroid by q and moved so
+// that its centroid lands on the given centre; rot_vec = rotvec(q).
+template <class T>
+Pose<T> pose_from_vec(const Ligand& L, const float* x0, const float* dihe0, const T* pv) {
+    Pose<T> p;
+    p.x.assign(x0, x0 + 3 * L.natom);
+    p.dihe.assign(dihe0, dihe0 + L.ntors);
+    T s[3] = {0, 0, 0};
+    for (int a = 0; a < L.natom; a++) for (int d = 0; d < 3; d++) s[d] += p.x[3 * a + d];
+    for (int d = 0; d < 3; d++) p.c[d] = s[d] / T(L.natom);
+    for (int t = 0; t < L.ntors; t++) apply_torsion(L, p, t, pv[7 + t] - p.dihe[t]);
+    T q[4] = {pv[3], pv[4], pv[5], pv[6]};
+    T c_old[3] = {p.c[0], p.c[1], p.c[2]};
+    for (int a = 0; a < L.natom; a++) {
+        T w[3] = {p.x[3 * a] - c_old[0], p.x[3 * a + 1] - c_old[1], p.x[3 * a + 2] - c_old[2]};
+        quat_rotate(q, w);
+        for (int d = 0; d < 3; d++) p.x[3 * a + d] = w[d] + pv[d];
+    }
+    for (int d = 0; d < 3; d++) p.c[d] = pv[d];
+    quat_to_rotvec(q, p.rv);
+    return p;
+}
+
+template <class F>
+void parallel_for(int n, int nthreads, F&& fn) {
+    if (nthreads <= 1 || n <= 1) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                int i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+#define UD2O_BOTH(MACRO) MACRO(double, f64) MACRO(float, f32)
+
+extern "C" {
+
+void ud2o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox4x32_10(ctr, key, out);
+}
+
+void ud2o_chain_blocks(uint32_t seed, uint32_t lig, uint32_t chain, int n, uint32_t* out) {
+    ChainRng r(seed, lig, chain);
+    for (int i = 0; i < n; i++) r.next(out + 4 * i);
+}
+
+double ud2o_eval_ef(double d, int t1, int t2, double* dedd) { return eval_ef<double>(d, t1, t2, dedd); }
+void ud2o_eval_decomp(double d, int t1, int t2, double* out5) { eval_decomp<double>(d, t1, t2, out5); }
+
+float ud2o_rmsd_f32(const float* a, const float* b, const int32_t* types, int natom) {
+    return rmsd_f32(a, b, types, natom);
+}
+
+// Greedy leader clustering of npose poses of one ligand; returns the number of
+// leaders written to out_leaders (selection order).
+int ud2o_cluster(int npose, const float* energies, const float* coords, const int32_t* types, int natom, float limit,
+                 int32_t* out_leaders) {
+    std::vector<int> l = cluster_leaders(npose, energies, coords, types, natom, limit);
+    for (size_t i = 0; i < l.size(); i++) out_leaders[i] = l[i];
+    return int(l.size());
+}
+
+int ud2o_filter(int nclustered, const int32_t* clustered, const float* energies, float energy_range, int num_pose,
+                int32_t* out) {
+    std::vector<int> c(clustered, clustered + nclustered);
+    std::vector<int> f = filter_poses(c, energies, energy_range, num_pose);
+    for (size_t i = 0; i < f.size(); i++) out[i] = f[i];
+    return int(f.size());
+}
+
+// Grid maps for the listed probe types: out[k][iz][iy][ix] at origin + spacing*(i).
+void ud2o_grid_build(int rec_natom, const float* rec_xyz, const int32_t* rec_types, const int32_t n[3],
+                     const float origin[3], float spacing, int ntypes, const int32_t* probe_types, float* out,
+                     int nthreads) {
+    Receptor R;
+    R.natom = rec_natom; R.xyz = rec_xyz; R.types = rec_types;
+    size_t plane = size_t(n[0]) * n[1];
+    int rows = ntypes * n[2];
+    parallel_for(rows, nthreads, [&](int row) {
+        int k = row / n[2], iz = row % n[2];
+        for (int iy = 0; iy < n[1]; iy++)
+            for (int ix = 0; ix < n[0]; ix++) {
+                double p[3] = {double(origin[0]) + double(spacing) * ix, double(origin[1]) + double(spacing) * iy,
+                               double(origin[2]) + double(spacing) * iz};
+                out[size_t(k) * plane * n[2] + size_t(iz) * plane + size_t(iy) * n[0] + ix] =
+                    float(grid_point_value<double>(R, probe_types[k], p));
+            }
+    });
+}
+
+// Grid-map values at arbitrary points (FP64), for spot checks of a full-size map.
+void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_types, int probe_type, int npts,
+                      const double* pts, double* out) {
+    Receptor R;
+    R.natom = rec_natom; R.xyz = rec_xyz; R.types = rec_types;
+    for (int i = 0; i < npts; i++) out[i] = grid_point_value<double>(R, probe_type, pts + 3 * i);
+}
+
+#define DEF_NUMERIC(T, SFX)                                                                                         \
+    T ud2o_energy_forces_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T v_cap, T slope, T* f,     \
+                               T* terms3) {                                                                        \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Terms<T> tm;                                                                                                \
+        T e = energy_forces<T>(L, x, S.scene, v_cap, slope, f, &tm);                                               \
+        if (terms3) { terms3[0] = tm.inter; terms3[1] = tm.intra; terms3[2] = tm.penalty; }                        \
+        return e;                                                                                                   \
+    }                                                                                                               \
+    T ud2o_energy_grad_##SFX(const ud2o_ligand* l, const T* x, const T* c, const T* rv, const T* dihe,             \
+                             const ud2o_scene* s, T v_cap, T slope, T* g, T* terms3) {                             \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(0));                                                          \
+        std::vector<T> fb;                                                                                          \
+        Terms<T> tm;                                                                                                \
+        T e = energy_grad<T>(L, p, S.scene, v_cap, slope, g, fb, &tm);                                             \
+        if (terms3) { terms3[0] = tm.inter; terms3[1] = tm.intra; terms3[2] = tm.penalty; }                        \
+        return e;                                                                                                   \
+    }                                                                                                               \
+    void ud2o_apply_torsion_##SFX(const ud2o_ligand* l, T* x, T* c, T* rv, T* dihe, int t, T inc) {                \
+        Ligand L = to_lig(l);                                                                                       \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(0));                                                          \
+        apply_torsion<T>(L, p, t, inc);                                                                             \
+        store_pose<T>(L, p, x, c, rv, dihe, nullptr);                                                               \
+    }                                                                                                               \
+    void ud2o_apply_update_##SFX(const ud2o_ligand* l, T* x, T* c, T* rv, T* dihe, const T* g, T alpha) {          \
+        Ligand L = to_lig(l);                                                                                       \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(0));                                                          \
+        apply_update<T>(L, p, g, alpha);                                                                            \
+        store_pose<T>(L, p, x, c, rv, dihe, nullptr);                                                               \
+    }                                                                                                               \
+    void ud2o_pose_from_vec_##SFX(const ud2o_ligand* l, const float* x0, const float* dihe0, const T* pv, T* x,     \
+                                  T* c, T* rv, T* dihe) {                                                           \
+        Ligand L = to_lig(l);                                                                                       \
+        Pose<T> p = pose_from_vec<T>(L, x0, dihe0, pv);                                                             \
+        store_pose<T>(L, p, x, c, rv, dihe, nullptr);                                                               \
+    }                                                                                                               \
+    /* E + gradient of n pose vectors; returns the wall time in seconds. */                                        \
+    double ud2o_score_posevecs_##SFX(const ud2o_ligand* l, const float* x0, const float* dihe0,                    \
+                                     const ud2o_scene* s, int n, const T* pvs, T v_cap, T slope, T* e, T* g,       \
+                                     int nthreads) {                                                                \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        int dimv = 7 + L.ntors, dimg = 6 + L.ntors;                                                                 \
+        auto t0 = std::chrono::steady_clock::now();                                                                 \
+        parallel_for(n, nthreads, [&](int i) {                                                                      \
+            thread_local std::vector<T> fb;                                                                         \
+            Pose<T> p = pose_from_vec<T>(L, x0, dihe0, pvs + size_t(i) * dimv);                                     \
+            e[i] = energy_grad<T>(L, p, S.scene, v_cap, slope, g + size_t(i) * dimg, fb);                           \
+        });                                                                                                         \
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();                       \
+    }                                                                                                               \
+    void ud2o_line_search_##SFX(const ud2o_ligand* l, const ud2o_scene* s, const T* x, const T* c, const T* rv,    \
+                                const T* dihe, const T* g, const T* p, T e0, T v_cap, T slope, T* x_out, T* c_out, \
+                                T* rv_out, T* dihe_out, T* g_out, T* e_out, T* alpha_out) {                        \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> xp = load_pose<T>(L, x, c, rv, dihe, T(0)), xn;                                                     \
+        std::vector<T> fb;                                                                                          \
+        line_search<T>(L, S.scene, xp, g, p, e0, v_cap, slope, xn, g_out, e_out, alpha_out, fb);                   \
+        store_pose<T>(L, xn, x_out, c_out, rv_out, dihe_out, nullptr);                                              \
+    }                                                                                                               \
+    /* In-place BFGS of one pose; returns the number of energy evaluations. */                                      \
+    int ud2o_bfgs_##SFX(const ud2o_ligand* l, const ud2o_scene* s, T* x, T* c, T* rv, T* dihe, T* energy,          \
+                        int max_steps, T v_cap, T slope) {                                                          \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, *energy);                                                       \
+        std::vector<T> fb;                                                                                          \
+        int ev = 0;                                                                                                 \
+        bfgs<T>(L, S.scene, p, max_steps, v_cap, slope, fb, &ev);                                                   \
+        store_pose<T>(L, p, x, c, rv, dihe, energy);                                                                \
+        return ev;                                                                                                  \
+    }                                                                                                               \
+    void ud2o_randomize_##SFX(const ud2o_ligand* l, const ud2o_scene* s, T* x, T* c, T* rv, T* dihe,               \
+                              uint32_t seed, int lig_index, int chain, int nchain) {                                \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(999));                                                        \
+        ChainRng rng(seed, uint32_t(lig_index), uint32_t(chain));                                                   \
+        randomize<T>(L, S.scene.box, p, chain, nchain, rng);                                                        \
+        store_pose<T>(L, p, x, c, rv, dihe, nullptr);                                                               \
+    }                                                                                                               \
+    void ud2o_mutate_##SFX(const ud2o_ligand* l, const ud2o_scene* s, T* x, T* c, T* rv, T* dihe, uint32_t seed,   \
+                           int lig_index, int chain, int skip_draws) {                                              \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(999));                                                        \
+        ChainRng rng(seed, uint32_t(lig_index), uint32_t(chain));                                                   \
+        rng.draw = uint32_t(skip_draws);                                                                            \
+        mutate<T>(L, S.scene.box, p, rng);                                                                          \
+        store_pose<T>(L, p, x, c, rv, dihe, nullptr);                                                               \
+    }                                                                                                               \
+    /* nchain MC chains of one ligand from one start pose (threads over chains).                                    \
+       Outputs per chain: best pose and its energy; returns the wall time (s). */                                   \
+    double ud2o_mc_##SFX(const ud2o_ligand* l, const ud2o_scene* s, const T* x, const T* c, const T* rv,           \
+                         const T* dihe, int lig_index, int nchain, int chain_begin, int chain_count, int mc_steps,  \
+                         int opt_steps, int randomize_flag, uint32_t seed, T* out_x, T* out_c, T* out_rv,           \
+                         T* out_dihe, T* out_e, long long* out_evals, int nthreads) {                               \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        Pose<T> start = load_pose<T>(L, x, c, rv, dihe, T(999));                                                    \
+        McParams P;                                                                                                 \
+        P.mc_steps = mc_steps; P.opt_steps = opt_steps; P.randomize = randomize_flag; P.seed = seed;               \
+        std::vector<long long> ev(chain_count, 0);                                                                  \
+        auto t0 = std::chrono::steady_clock::now();                                                                 \
+        parallel_for(chain_count, nthreads, [&](int k) {                                                            \
+            thread_local std::vector<T> fb;                                                                         \
+            int ch = chain_begin + k;                                                                               \
+            Pose<T> b = mc_chain<T>(L, S.scene, start, lig_index, ch, nchain, P, fb, &ev[k]);                       \
+            store_pose<T>(L, b, out_x + size_t(k) * 3 * L.natom, out_c + 3 * size_t(k), out_rv + 3 * size_t(k),    \
+                          out_dihe + size_t(k) * L.ntors, out_e + k);                                               \
+        });                                                                                                         \
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();                   \
+        if (out_evals) { long long s_ = 0; for (auto v : ev) s_ += v; *out_evals = s_; }                            \
+        return dt;                                                                                                  \
+    }                                                                                                               \
+    void ud2o_final_score_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T* out3) {                  \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        final_score<T>(L, x, S.rec, S.scene.box, out3);                                                             \
+    }                                                                                                               \
+    /* Whole docking task of one ligand, DockTask::run (E/src/task/DockTask.cpp:39-123): MC chains,                \
+       clustering, refinement with the slope ramp (:155-186), filter (:188-220) and final scoring with             \
+       the affinity of :224-282. out_x[k] holds the k-th output pose, out_e[k*7] its energy vector                  \
+       [affinity, total, intra, inter, penalty, conf_indep, bias]. Returns the number of output poses and          \
+       writes the wall time (s) of the timed region to *seconds. */                                                 \
+    int ud2o_dock_ligand_##SFX(const ud2o_ligand* l, const ud2o_scene* s, const T* x, const T* c, const T* rv,     \
+                               const T* dihe, int lig_index, int nchain, int mc_steps, int opt_steps,               \
+                               int refine_steps, int randomize_flag, uint32_t seed, float rmsd_limit,               \
+                               float energy_range, int num_pose, T* out_x, T* out_e, double* seconds,               \
+                               long long* out_evals, int nthreads) {                                                \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        auto t0 = std::chrono::steady_clock::now();                                                                 \
+        Pose<T> start = load_pose<T>(L, x, c, rv, dihe, T(999));                                                    \
+        McParams P;                                                                                                 \
+        P.mc_steps = mc_steps; P.opt_steps = opt_steps; P.randomize = randomize_flag; P.seed = seed;               \
+        std::vector<Pose<T>> res(nchain);                                                                           \
+        std::vector<long long> ev(nchain, 0);                                                                       \
+        parallel_for(nchain, nthreads, [&](int k) {                                                                 \
+            thread_local std::vector<T> fb;                                                                         \
+            res[k] = mc_chain<T>(L, S.scene, start, lig_index, k, nchain, P, fb, &ev[k]);                           \
+        });                                                                                                         \
+        std::vector<float> ef(nchain), xf(size_t(nchain) * 3 * L.natom);                                            \
+        for (int k = 0; k < nchain; k++) {                                                                          \
+            ef[k] = float(res[k].energy);                                                                           \
+            for (int i = 0; i < 3 * L.natom; i++) xf[size_t(k) * 3 * L.natom + i] = float(res[k].x[i]);             \
+        }                                                                                                           \
+        std::vector<int> leaders = cluster_leaders(nchain, ef.data(), xf.data(), L.types, L.natom, rmsd_limit);     \
+        for (int pr = 0; pr < refine_steps; pr++) {                                                                 \
+            double sl = 100.0 * std::pow(10.0, 2.0 * pr);                                                           \
+            T slope = sl > double(kPenaltySlope) ? T(kPenaltySlope) : T(sl);                                        \
+            parallel_for(int(leaders.size()), nthreads, [&](int k) {                                                \
+                thread_local std::vector<T> fb;                                                                     \
+                int ev_ = 0;                                                                                        \
+                bfgs<T>(L, S.scene, res[leaders[k]], 2, T(kVCapAuthentic), slope, fb, &ev_);                        \
+            });                                                                                                     \
+        }                                                                                                           \
+        for (int k = 0; k < nchain; k++) ef[k] = float(res[k].energy);                                              \
+        std::vector<int> keep = filter_poses(leaders, ef.data(), energy_range, num_pose);                           \
+        T sc[3];                                                                                                    \
+        T intra_r1 = 0;                                                                                             \
+        for (size_t k = 0; k < keep.size(); k++) {                                                                  \
+            const Pose<T>& p = res[keep[k]];                                                                        \
+            final_score<T>(L, p.x.data(), S.rec, S.scene.box, sc);                                                  \
+            if (k == 0) intra_r1 = sc[0];                                                                           \
+            T total = sc[0] + sc[1] + sc[2];                                                                        \
+            T inter = total - intra_r1;                                                                             \
+            T aff = conf_indep<T>(inter, L.ntors);                                                                  \
+            T* e7 = out_e + 7 * k;                                                                                  \
+            e7[0] = aff; e7[1] = total; e7[2] = sc[0]; e7[3] = sc[1]; e7[4] = sc[2]; e7[5] = aff - inter;          \
+            e7[6] = 0;                                                                                              \
+            std::copy(p.x.begin(), p.x.end(), out_x + k * 3 * L.natom);                                             \
+        }                                                                                                           \
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();      \
+        if (out_evals) { long long s_ = 0; for (auto v : ev) s_ += v; *out_evals = s_; }                            \
+        return int(keep.size());                                                                                    \
+    }
+
+UD2O_BOTH(DEF_NUMERIC)
+
+}  // extern "C"

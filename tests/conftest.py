@@ -1,0 +1,47 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "uni-dock2_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
+
+
+@pytest.fixture(scope="session")
+def receptor():
+    from unidock_engine import synthetic as S
+    return S.make_receptor()
+
+
+@pytest.fixture(scope="session")
+def box():
+    from unidock_engine import synthetic as S
+    return S.docking_box()
+
+
+@pytest.fixture(scope="session")
+def lig_c2():
+    from unidock_engine import synthetic as S
+    return S.config2_ligand().ligand
+
+
+@pytest.fixture(scope="session")
+def lig_c0():
+    from unidock_engine import synthetic as S
+    return S.config0_ligand().ligand
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from unidock_engine import _native as N
+    N.check(N.lib().ud2_set_device(0), "ud2_set_device")
+    return N

@@ -1,0 +1,307 @@
+"""HIP engine vs the CPU oracle (FP64) through the C ABI.
+
+Tolerances: energies and gradients within 1e-4 relative (floor 1.0) of the
+FP64 oracle on the same inputs and the same FP32 grid maps; coordinates from
+the pose update within 1e-4 A; BFGS final energies within 1e-3 kcal/mol;
+clustering membership bit-exact.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VP = C.c_void_p
+
+
+def dptr(t):
+    return VP(t.data_ptr())
+
+
+def rel_err(a, b, floor=1.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def grad_err(g, gref):
+    g = np.asarray(g, np.float64)
+    gref = np.asarray(gref, np.float64)
+    scale = np.maximum(np.abs(gref).max(axis=-1, keepdims=True), 1.0)
+    return np.abs(g - gref) / scale
+
+
+@pytest.fixture(scope="module")
+def scene(gpu, receptor, box):
+    from unidock_engine import synthetic as S
+    sc = gpu.Scene(receptor, box)
+    sc.build_grids(S.GRID_SPACING, [2, 3, 4, 5, 6, 10, 11, 15])
+    return sc
+
+
+@pytest.fixture(scope="module")
+def oracle_scenes(scene, receptor, box):
+    n, org, sp, nt = scene.grid_info()
+    maps = scene.download_maps()
+    slot = [-1] * 21
+    for k, t in enumerate([2, 3, 4, 5, 6, 10, 11, 15]):
+        slot[t] = k
+    return {
+        "direct": O.Scene(receptor, box, "direct"),
+        "grid": O.Scene(receptor, box, "grid", maps=maps, grid_n=n, grid_origin=org, grid_spacing=sp,
+                        slot_of_type=slot),
+    }
+
+
+def test_grid_maps_match_oracle(gpu, receptor):
+    from unidock_engine import topology as T
+    small_box = T.Box.from_center_size((12.5 + 11.0, -3.0, 7.25), (6.0, 6.0, 6.0))
+    sc = gpu.Scene(receptor, small_box)
+    probes = [2, 3, 4, 5, 6, 10, 11, 15]
+    sc.build_grids(0.5, probes)
+    n, org, sp, nt = sc.grid_info()
+    assert list(n) == [13, 13, 13] and nt == 8
+    got = sc.download_maps()
+    ref = O.grid_build(receptor, n, org, sp, probes, nthreads=8)
+    assert np.abs(ref).max() > 1.0  # the probe box touches the receptor shell
+    err = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1.0)
+    assert err.max() < 1e-5, err.max()
+
+
+@pytest.mark.parametrize("mode", ["direct", "grid"])
+@pytest.mark.parametrize("v_cap", [10.0, 1e30])
+def test_score_posevecs_matches_oracle(gpu, scene, oracle_scenes, lig_c2, box, mode, v_cap):
+    import torch
+    from unidock_engine import synthetic as S
+    ls = gpu.LigSet([lig_c2])
+    n = 256
+    pv = S.random_pose_vectors(n, lig_c2.ntors, box, seed=11)
+    # half of the poses kept well inside the box, near the receptor shell
+    pv[n // 2:, 0:3] = np.asarray(S.POCKET_CENTER, np.float32) + np.random.default_rng(2).uniform(-6, 6, (n // 2, 3))
+    pv_d = torch.as_tensor(pv, device="cuda")
+    e_d = torch.zeros(n, device="cuda")
+    g_d = torch.zeros((n, 6 + lig_c2.ntors), device="cuda")
+    m = gpu.MODE_GRID if mode == "grid" else gpu.MODE_DIRECT
+    gpu.check(gpu.lib().ud2_score_posevecs(scene.h, ls.h, 0, n, dptr(pv_d), dptr(e_d), dptr(g_d), m, v_cap, 1e6,
+                                           gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    e_ref, g_ref, _ = O.score_posevecs(O.Topo(lig_c2), oracle_scenes[mode], pv, v_cap=v_cap, slope=1e6, nthreads=8)
+    e = e_d.cpu().numpy()
+    g = g_d.cpu().numpy()
+    assert np.isfinite(e).all() and np.isfinite(g).all()
+    assert rel_err(e, e_ref).max() < 1e-4
+    assert grad_err(g, g_ref).max() < 1e-4
+
+
+def _oracle_pose(pb, lig, chain):
+    p = pb.pose(lig, chain)
+    return O.PoseArr(p["x"], p["c"], p["rv"], p["dihe"], p["e"])
+
+
+def test_pose_update_matches_oracle(gpu, lig_c2, box):
+    import torch
+    from unidock_engine import synthetic as S
+    ls = gpu.LigSet([lig_c2])
+    nchain = 32
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    topo = O.Topo(lig_c2)
+    rng = np.random.default_rng(3)
+    dim = 6 + lig_c2.ntors
+    dirs = np.zeros((nchain, 64), np.float32)
+    dirs[:, :dim] = rng.normal(size=(nchain, dim)) * np.r_[[2.0] * 3, [0.7] * 3, [1.5] * lig_c2.ntors]
+    starts = []
+    for c in range(nchain):
+        p0 = pb.pose(0, c)
+        starts.append(O.PoseArr(p0["x"], p0["c"], p0["rv"], p0["dihe"]))
+    sel = torch.arange(nchain, dtype=torch.int32, device="cuda")
+    d_dirs = torch.as_tensor(dirs, device="cuda")
+    gpu.check(gpu.lib().ud2_pose_update(ls.h, pb.ref, dptr(sel), nchain, dptr(d_dirs), 0.75, gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    for c in range(nchain):
+        ref = starts[c].copy()
+        O.apply_update(topo, ref, dirs[c, :dim].astype(np.float64), 0.75)
+        got = pb.pose(0, c)
+        assert np.abs(got["x"] - ref.x).max() < 1e-4
+        assert np.abs(got["c"] - ref.c).max() < 1e-4
+        assert np.abs(got["rv"] - ref.rv).max() < 1e-5
+        assert np.abs(got["dihe"] - ref.dihe[:lig_c2.ntors]).max() < 1e-5
+
+
+def test_line_search_one_atom_kat(gpu):
+    """test_bfgs.cpp:84-130: one C_H receptor atom at the origin, one C_H ligand atom at r = 1.5:
+    the line search from H = I takes alpha = 1 and x_new = x - g."""
+    import torch
+    from unidock_engine import topology as T
+    rec = T.Receptor(xyz=np.zeros((1, 3), np.float32), types=np.asarray([2], np.int32))
+    box = T.Box(lo=np.full(3, -30, np.float32), hi=np.full(3, 30, np.float32))
+    lig = T.make_ligand("one", np.asarray([[1.5, 0, 0]], np.float32), [2], [], [], [], [], np.zeros((0, 2)))
+    sc = gpu.Scene(rec, box)
+    ls = gpu.LigSet([lig])
+    pb = gpu.PoseBuffers(ls, 1)
+    pb.init()
+    sel = torch.zeros(1, dtype=torch.int32, device="cuda")
+    e = torch.zeros(1, device="cuda")
+    g = torch.zeros((1, 64), device="cuda")
+    gpu.check(gpu.lib().ud2_energy_grad(sc.h, ls.h, pb.ref, dptr(sel), 1, 0, 1e30, 1e6, dptr(e), dptr(g),
+                                        gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    e_ref, dedd = O.eval_ef(1.5 - 3.8, 2, 2)
+    assert abs(float(e[0]) - e_ref) < 1e-4
+    assert abs(float(g[0, 0]) - dedd) < 1e-4 and abs(float(g[0, 1])) < 1e-4
+    g0 = g[0, :6].cpu().numpy().copy()
+    alpha = torch.zeros(1, device="cuda")
+    gpu.check(gpu.lib().ud2_line_search(sc.h, ls.h, pb.ref, dptr(sel), 1, 0, 1e30, 1e6, dptr(alpha), dptr(g),
+                                        gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    p = pb.pose(0, 0)
+    assert abs(float(alpha[0]) - 1.0) < 1e-6
+    assert abs(p["x"][0] - (1.5 - g0[0])) < 1e-4
+    r = float(np.linalg.norm(p["x"]))
+    assert abs(p["e"] - O.eval_ef(r - 3.8, 2, 2)[0]) < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["direct", "grid"])
+def test_energy_grad_and_bfgs_match_oracle(gpu, scene, oracle_scenes, lig_c2, box, mode):
+    import torch
+    from unidock_engine import synthetic as S
+    ls = gpu.LigSet([lig_c2])
+    nchain = 16
+    pb = gpu.PoseBuffers(ls, nchain)
+    topo = O.Topo(lig_c2)
+    rng = np.random.default_rng(7)
+    pv = S.random_pose_vectors(nchain, lig_c2.ntors, box, seed=21)
+    pv[:, 0:3] = np.asarray(S.POCKET_CENTER, np.float32) + rng.uniform(-7, 7, (nchain, 3))
+    pv[:, 7:] = lig_c2.dihedrals[None, :]  # input torsions, random rigid placement near the receptor wall
+    starts = []
+    for c in range(nchain):
+        p = O.pose_from_vec(topo, pv[c].astype(np.float64))
+        pb.set_pose(0, c, p.x, p.c, p.rv, p.dihe[:lig_c2.ntors])
+        starts.append(O.PoseArr(pb.pose(0, c)["x"], p.c, p.rv, p.dihe[:lig_c2.ntors]))
+    sel = torch.arange(nchain, dtype=torch.int32, device="cuda")
+    m = gpu.MODE_GRID if mode == "grid" else gpu.MODE_DIRECT
+    e = torch.zeros(nchain, device="cuda")
+    g = torch.zeros((nchain, 64), device="cuda")
+    gpu.check(gpu.lib().ud2_energy_grad(scene.h, ls.h, pb.ref, dptr(sel), nchain, m, 1000.0, 1e6, dptr(e), dptr(g),
+                                        gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    dim = 6 + lig_c2.ntors
+    for c in range(nchain):
+        e_ref, g_ref, _ = O.energy_grad(topo, starts[c], oracle_scenes[mode], v_cap=1000.0, slope=1e6)
+        assert rel_err(float(e[c]), e_ref) < 1e-4
+        assert grad_err(g[c, :dim].cpu().numpy(), g_ref).max() < 1e-4
+    # BFGS from the same starts. Short runs (1 and 2 iterations) must agree on every
+    # start; the 8-iteration final energy must agree on every well-conditioned start,
+    # i.e. one where the oracle's own FP32 and FP64 runs end within 1e-4 of each other
+    # (elsewhere a line-search branch flips between float and double arithmetic).
+    def run_gpu(steps):
+        for c in range(nchain):
+            s0 = starts[c]
+            pb.set_pose(0, c, s0.x, s0.c, s0.rv, s0.dihe[:lig_c2.ntors])
+        gpu.check(gpu.lib().ud2_optimize(scene.h, ls.h, pb.ref, dptr(sel), nchain, m, steps, 1000.0, 1e6, dptr(e),
+                                         gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        return e.cpu().numpy().astype(np.float64)
+
+    def run_oracle(steps, sfx):
+        out = []
+        for c in range(nchain):
+            ref = starts[c].copy()
+            if sfx == "f32":
+                ref = O.PoseArr(ref.x, ref.c, ref.rv, ref.dihe, 999.0, np.float32)
+            O.bfgs(topo, oracle_scenes[mode], ref, steps, v_cap=1000.0, slope=1e6, sfx=sfx)
+            out.append(float(ref.energy[0]))
+        return np.asarray(out)
+
+    for steps in (1, 2):
+        d = np.abs(run_gpu(steps) - run_oracle(steps, "f64"))
+        assert d.max() < 1e-3, (steps, d)
+    steps = 8
+    e64, e32, eg = run_oracle(steps, "f64"), run_oracle(steps, "f32"), run_gpu(steps)
+    well = np.abs(e64 - e32) < 1e-4
+    assert well.mean() >= 0.5, (e64, e32)
+    d = np.abs(eg - e64)
+    assert d[well].max() < 1e-3, (d, well)
+
+
+def test_cluster_bit_exact(gpu, lig_c2):
+    import torch
+    ls = gpu.LigSet([lig_c2, lig_c2])
+    nchain = 96
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    rng = np.random.default_rng(5)
+    n3 = 3 * lig_c2.natom
+    base = lig_c2.coords.reshape(-1).astype(np.float32)
+    xs = (base[None, :] + rng.normal(scale=0.6, size=(2 * nchain, n3))).astype(np.float32)
+    es = rng.normal(size=2 * nchain).astype(np.float32)
+    es[5] = es[9]  # a tie, resolved by index
+    pb.x[:] = torch.as_tensor(xs.reshape(-1), device="cuda")
+    pb.e[:] = torch.as_tensor(es, device="cuda")
+    leaders = torch.zeros((2, nchain), dtype=torch.int32, device="cuda")
+    nlead = torch.zeros(2, dtype=torch.int32, device="cuda")
+    for limit in (0.5, 1.0, 1.3):
+        gpu.check(gpu.lib().ud2_cluster(ls.h, pb.ref, limit, dptr(leaders), dptr(nlead), gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        for l in range(2):
+            ref = O.cluster(es[l * nchain:(l + 1) * nchain], xs[l * nchain:(l + 1) * nchain], lig_c2.types, limit)
+            got = leaders[l, :int(nlead[l])].cpu().numpy()
+            assert np.array_equal(got, ref), (limit, l)
+
+
+def test_final_score_matches_oracle(gpu, scene, oracle_scenes, lig_c2, box):
+    import torch
+    from unidock_engine import synthetic as S
+    ls = gpu.LigSet([lig_c2])
+    nchain = 8
+    pb = gpu.PoseBuffers(ls, nchain)
+    topo = O.Topo(lig_c2)
+    pv = S.random_pose_vectors(nchain, lig_c2.ntors, box, seed=4)
+    for c in range(nchain):
+        p = O.pose_from_vec(topo, pv[c].astype(np.float64))
+        pb.set_pose(0, c, p.x, p.c, p.rv, p.dihe[:lig_c2.ntors])
+    sel = torch.arange(nchain, dtype=torch.int32, device="cuda")
+    out = torch.zeros((nchain, 3), device="cuda")
+    gpu.check(gpu.lib().ud2_final_score(scene.h, ls.h, pb.ref, dptr(sel), nchain, dptr(out), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    for c in range(nchain):
+        ref = O.final_score(topo, pb.pose(0, c)["x"], oracle_scenes["direct"])
+        assert rel_err(out[c].cpu().numpy(), ref).max() < 1e-4
+
+
+def test_mc_distribution_matches_oracle(gpu, scene, oracle_scenes, lig_c0):
+    """Plain MC (randomise + 8 steps x BFGS(8)) in grid mode, judged as the north star
+    asks: the distribution of the best score over 32 seeds (16 chains each) must match
+    the FP64 oracle's (two-sample KS p > 0.01, medians within 0.25 kcal/mol). Chains
+    use the same Philox streams as the oracle; individual trajectories may still split
+    where float and double round a decision differently."""
+    import torch
+    from scipy.stats import ks_2samp
+    ls = gpu.LigSet([lig_c0])
+    nchain, seeds = 16, 32
+    pb = gpu.PoseBuffers(ls, nchain)
+    topo = O.Topo(lig_c0)
+    start = O.PoseArr.initial(lig_c0)
+    best_gpu, best_ref = [], []
+    for seed in range(seeds):
+        pb.init()
+        prm = gpu.McParams(mc_steps=8, opt_steps=8, randomize=1, seed=seed, mode=gpu.MODE_GRID, lig_index_base=0)
+        gpu.check(gpu.lib().ud2_mc(scene.h, ls.h, pb.ref, C.byref(prm), gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        e = pb.e[:nchain].cpu().numpy().astype(np.float64)
+        assert np.isfinite(e).all()
+        res, _, _ = O.mc(topo, oracle_scenes["grid"], start, 0, nchain, 8, 8, seed, True, nthreads=8)
+        best_gpu.append(e.min())
+        best_ref.append(res["e"].min())
+        if seed == 0:
+            # every chain's pose is a real pose of the ligand: bond lengths kept by the kinematics
+            x = pb.x[:nchain * 3 * lig_c0.natom].cpu().numpy().reshape(nchain, lig_c0.natom, 3)
+            for i, j in lig_c0.axis:
+                d0 = np.linalg.norm(lig_c0.coords[i] - lig_c0.coords[j])
+                assert np.abs(np.linalg.norm(x[:, i] - x[:, j], axis=1) - d0).max() < 1e-3
+    best_gpu, best_ref = np.asarray(best_gpu), np.asarray(best_ref)
+    assert ks_2samp(best_gpu, best_ref).pvalue > 0.01, (np.sort(best_gpu), np.sort(best_ref))
+    assert abs(np.median(best_gpu) - np.median(best_ref)) < 0.25

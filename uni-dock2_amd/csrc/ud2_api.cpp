@@ -1,0 +1,737 @@
+// Host side of libud2.so: the C ABI declared in include/ud2.h.
+//
+// Owns the device copies of the receptor / grid maps (ud2_scene) and of a
+// batch of ligand topologies (ud2_ligset), validates every input before a
+// kernel sees it, and launches the kernels of ud2_kernels.hip. Errors become
+// return codes with a per-thread message; nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ud2.h"
+#include "ud2_internal.h"
+
+using namespace ud2;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define UD2_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(UD2_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+const float kVdwHost[kNumTypes] = {1.1f, 1.92f, 1.9f, 1.9f, 1.8f, 1.8f, 1.8f, 1.8f, 1.7f, 1.7f, 1.7f,
+                                   1.7f, 1.5f, 2.2f, 2.1f, 2.0f, 1.8f, 2.0f, 2.2f, 2.3f, 1.2f};
+const int kFlagsHost[kNumTypes] = {0, 0, 1, 0, 0, 4, 2, 6, 0, 4, 2, 6, 1, 0, 0, 0, 1, 1, 1, 0, 4};
+
+int pair_flags_host(int ti, int tj) {
+    int a = kFlagsHost[ti], b = kFlagsHost[tj];
+    int hyd = a & b & 1;
+    int hb = (((a >> 1) & (b >> 2)) | ((a >> 2) & (b >> 1))) & 1;
+    return hyd | (hb << 1);
+}
+
+template <class T>
+int upload(T** d, const std::vector<T>& h) {
+    size_t n = std::max<size_t>(h.size(), 1);
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(d), n * sizeof(T)));
+    if (!h.empty()) UD2_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return UD2_OK;
+}
+
+int n_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+}
+
+int odd_ld(int dim) { return (dim % 2 == 1) ? dim : dim + 1; }
+
+}  // namespace
+
+struct ud2_scene {
+    int device = 0;
+    float4* d_rec = nullptr;
+    int32_t nrec = 0;
+    float box_lo[3], box_hi[3];
+    float* d_maps = nullptr;
+    int32_t gn[3] = {0, 0, 0};
+    float gorigin[3] = {0, 0, 0};
+    float gspacing = 0;
+    std::vector<int32_t> probe_types;
+    int32_t slot_of_type[kNumTypes];
+    int32_t* d_probe = nullptr;
+
+    SceneDev view() const {
+        SceneDev s;
+        s.rec = d_rec;
+        s.nrec = nrec;
+        for (int d = 0; d < 3; d++) {
+            s.box_lo[d] = box_lo[d];
+            s.box_hi[d] = box_hi[d];
+            s.gn[d] = gn[d];
+            s.gorigin[d] = gorigin[d];
+        }
+        s.maps = d_maps;
+        s.gspacing = gspacing;
+        s.ginv = gspacing > 0 ? 1.0f / gspacing : 0.0f;
+        for (int t = 0; t < kNumTypes; t++) s.slot_of_type[t] = slot_of_type[t];
+        return s;
+    }
+};
+
+struct ud2_ligset {
+    int device = 0;
+    int32_t n_lig = 0;
+    std::vector<DevLig> h_ligs;
+    std::vector<uint8_t> has_slot_type;  // union of heavy ligand types, per XS type
+    DevLig* d_ligs = nullptr;
+    float4* d_x0 = nullptr;
+    int2* d_axis = nullptr;
+    float* d_dihe0 = nullptr;
+    int2* d_trange = nullptr;
+    float2* d_ranges = nullptr;
+    unsigned long long* d_mask = nullptr;
+    uint2* d_pairs = nullptr;
+    int32_t* d_counter = nullptr;
+    int32_t* d_order = nullptr;   // work order, n_lig entries
+    int32_t* d_sel_all = nullptr; // scratch for all-pose selections
+    int64_t sel_all_cap = 0;
+
+    LigSetDev view() const {
+        LigSetDev v;
+        v.ligs = d_ligs;
+        v.atom_x0 = d_x0;
+        v.tors_axis = d_axis;
+        v.tors_dihe0 = d_dihe0;
+        v.tors_range = d_trange;
+        v.ranges = d_ranges;
+        v.tors_mask = d_mask;
+        v.pairs = d_pairs;
+        return v;
+    }
+    int max_apl() const {
+        int m = 1;
+        for (auto& l : h_ligs) m = std::max(m, int(l.apl));
+        return m;
+    }
+    int max_dim() const {
+        int m = 6;
+        for (auto& l : h_ligs) m = std::max(m, 6 + int(l.ntors));
+        return m;
+    }
+};
+
+namespace {
+
+PoseBufs pose_view(const ud2_poses* P) {
+    PoseBufs b;
+    b.x = P->x_d;
+    b.c = P->c_d;
+    b.rv = P->rv_d;
+    b.dihe = P->dihe_d;
+    b.e = P->e_d;
+    b.nchain = P->nchain;
+    return b;
+}
+
+int check_poses(const ud2_ligset* ls, const ud2_poses* P) {
+    if (!ls) return fail(UD2_ERR_ARG, "null ligand set");
+    if (!P || !P->x_d || !P->c_d || !P->rv_d || !P->dihe_d || !P->e_d) return fail(UD2_ERR_ARG, "null pose buffer");
+    if (P->nchain <= 0) return fail(UD2_ERR_ARG, "nchain must be positive");
+    return UD2_OK;
+}
+
+int check_mode(const ud2_scene* sc, const ud2_ligset* ls, int32_t mode) {
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    if (mode != UD2_MODE_DIRECT && mode != UD2_MODE_GRID) return fail(UD2_ERR_ARG, "mode must be 0 (direct) or 1 (grid)");
+    if (mode == UD2_MODE_GRID) {
+        if (!sc->d_maps) return fail(UD2_ERR_STATE, "grid mode requested but the scene has no grid maps");
+        for (int t = 1; t < kNumTypes; t++)
+            if (ls->has_slot_type[t] && sc->slot_of_type[t] < 0)
+                return fail(UD2_ERR_STATE, "grid mode: no map for ligand atom type " + std::to_string(t));
+    }
+    return UD2_OK;
+}
+
+int ensure_sel(ud2_ligset* ls, int64_t n) {
+    if (n <= ls->sel_all_cap) return UD2_OK;
+    if (ls->d_sel_all) UD2_HIP(hipFree(ls->d_sel_all));
+    ls->d_sel_all = nullptr;
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&ls->d_sel_all), size_t(n) * sizeof(int32_t)));
+    ls->sel_all_cap = n;
+    return UD2_OK;
+}
+
+// Runs one opt_kernel operation over the selected poses: the selection is
+// regrouped by atoms-per-lane class and each class launched with its own LDS
+// size; per-pose outputs stay at the pose's position in sel_d.
+int run_opt(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
+            int32_t mode, int op, int32_t max_steps, float v_cap, float slope, float* out_e, float* out_g,
+            const float* dir, float alpha, hipStream_t st) {
+    if (n_sel <= 0) return UD2_OK;
+    if (!sel_d) return fail(UD2_ERR_ARG, "null selection");
+    std::vector<int32_t> h(n_sel);
+    UD2_HIP(hipMemcpy(h.data(), sel_d, size_t(n_sel) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int64_t npose = int64_t(ls->n_lig) * P->nchain;
+    std::vector<int32_t> ids, pos;
+    std::vector<int> start(kMaxApl + 2, 0), dmax(kMaxApl + 1, 6);
+    for (int apl = 1; apl <= kMaxApl; apl++) {
+        start[apl] = int(ids.size());
+        for (int32_t s = 0; s < n_sel; s++) {
+            const int32_t p = h[s];
+            if (p < 0 || p >= npose) return fail(UD2_ERR_ARG, "pose id out of range: " + std::to_string(p));
+            const DevLig& d = ls->h_ligs[p / P->nchain];
+            if (d.apl != apl) continue;
+            ids.push_back(p);
+            pos.push_back(s);
+            dmax[apl] = std::max(dmax[apl], 6 + int(d.ntors));
+        }
+    }
+    start[kMaxApl + 1] = int(ids.size());
+    int rc = ensure_sel(const_cast<ud2_ligset*>(ls), 2 * int64_t(n_sel));
+    if (rc) return rc;
+    UD2_HIP(hipStreamSynchronize(st));
+    UD2_HIP(hipMemcpy(ls->d_sel_all, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    UD2_HIP(hipMemcpy(ls->d_sel_all + n_sel, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    OptArgs A{};
+    A.L = ls->view();
+    SceneDev empty{};
+    A.S = sc ? sc->view() : empty;
+    A.P = pose_view(P);
+    A.mode = mode;
+    A.max_steps = max_steps;
+    A.v_cap = v_cap;
+    A.slope = slope;
+    A.out_e = out_e;
+    A.out_g = out_g;
+    A.dir = dir;
+    A.alpha = alpha;
+    A.op = op;
+    for (int apl = 1; apl <= kMaxApl; apl++) {
+        const int n = start[apl + 1] - start[apl];
+        if (n == 0) continue;
+        A.hdim_ld = odd_ld(dmax[apl]);
+        A.lds_floats_per_wave = lds_fixed_floats(apl) + dmax[apl] * A.hdim_ld;
+        A.sel = ls->d_sel_all + start[apl];
+        A.pos = ls->d_sel_all + n_sel + start[apl];
+        A.n_sel = n;
+        UD2_HIP(launch_opt(A, apl, st));
+    }
+    return UD2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ud2_version(void) { return "ud2-mi355x 0.1.0 (gfx950)"; }
+
+const char* ud2_last_error(void) { return g_err.c_str(); }
+
+int ud2_device_count(int32_t* n_h) {
+    if (!n_h) return fail(UD2_ERR_ARG, "null output");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *n_h = 0;
+        return fail(UD2_ERR_NODEV, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    *n_h = n;
+    return UD2_OK;
+}
+
+int ud2_set_device(int32_t id) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(UD2_ERR_NODEV, "no GPU device is visible");
+    if (id < 0 || id >= n) return fail(UD2_ERR_NODEV, "device id out of range");
+    UD2_HIP(hipSetDevice(id));
+    return UD2_OK;
+}
+
+int ud2_synchronize(void) {
+    UD2_HIP(hipDeviceSynchronize());
+    return UD2_OK;
+}
+
+// ------------------------------------------------------------------- scene
+int ud2_scene_create(const float* rec_xyz_h, const int32_t* rec_types_h, int32_t n_rec, const float box_lo_h[3],
+                     const float box_hi_h[3], ud2_scene** out) {
+    if (!out) return fail(UD2_ERR_ARG, "null output");
+    *out = nullptr;
+    if (n_rec < 0 || (n_rec > 0 && (!rec_xyz_h || !rec_types_h)) || !box_lo_h || !box_hi_h)
+        return fail(UD2_ERR_ARG, "bad receptor arguments");
+    for (int d = 0; d < 3; d++)
+        if (!(box_hi_h[d] > box_lo_h[d])) return fail(UD2_ERR_ARG, "box must have hi > lo on every axis");
+    std::vector<float4> rec;
+    rec.reserve(n_rec);
+    for (int i = 0; i < n_rec; i++) {
+        int t = rec_types_h[i];
+        if (t < 0 || t >= kNumTypes) return fail(UD2_ERR_ARG, "receptor atom type out of range");
+        if (t == 0) continue;  // hydrogens carry no Vina term
+        float4 v;
+        v.x = rec_xyz_h[3 * i];
+        v.y = rec_xyz_h[3 * i + 1];
+        v.z = rec_xyz_h[3 * i + 2];
+        std::memcpy(&v.w, &t, sizeof(int));
+        rec.push_back(v);
+    }
+    ud2_scene* sc = new ud2_scene();
+    hipGetDevice(&sc->device);
+    for (int d = 0; d < 3; d++) {
+        sc->box_lo[d] = box_lo_h[d];
+        sc->box_hi[d] = box_hi_h[d];
+    }
+    for (int t = 0; t < kNumTypes; t++) sc->slot_of_type[t] = -1;
+    sc->nrec = int32_t(rec.size());
+    int rc = upload(&sc->d_rec, rec);
+    if (rc) {
+        delete sc;
+        return rc;
+    }
+    *out = sc;
+    return UD2_OK;
+}
+
+static int set_probe_types(ud2_scene* sc, const int32_t* probe_types_h, int32_t n_types) {
+    if (n_types <= 0 || n_types > kNumTypes || !probe_types_h) return fail(UD2_ERR_ARG, "bad probe type list");
+    for (int t = 0; t < kNumTypes; t++) sc->slot_of_type[t] = -1;
+    sc->probe_types.assign(probe_types_h, probe_types_h + n_types);
+    for (int k = 0; k < n_types; k++) {
+        int t = probe_types_h[k];
+        if (t <= 0 || t >= kNumTypes) return fail(UD2_ERR_ARG, "probe type out of range (hydrogen has no map)");
+        if (sc->slot_of_type[t] >= 0) return fail(UD2_ERR_ARG, "duplicate probe type");
+        sc->slot_of_type[t] = k;
+    }
+    if (sc->d_probe) hipFree(sc->d_probe);
+    sc->d_probe = nullptr;
+    return upload(&sc->d_probe, sc->probe_types);
+}
+
+int ud2_scene_build_grids(ud2_scene* sc, float spacing, const int32_t* probe_types_h, int32_t n_types) {
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    if (!(spacing > 0.0f)) return fail(UD2_ERR_ARG, "spacing must be positive");
+    int rc = set_probe_types(sc, probe_types_h, n_types);
+    if (rc) return rc;
+    int64_t npt = 1;
+    for (int d = 0; d < 3; d++) {
+        float size = sc->box_hi[d] - sc->box_lo[d];
+        sc->gn[d] = int32_t(std::lround(double(size) / double(spacing))) + 1;
+        if (sc->gn[d] < 2) return fail(UD2_ERR_ARG, "box smaller than two grid points");
+        sc->gorigin[d] = sc->box_lo[d];
+        npt *= sc->gn[d];
+    }
+    sc->gspacing = spacing;
+    if (sc->d_maps) UD2_HIP(hipFree(sc->d_maps));
+    sc->d_maps = nullptr;
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&sc->d_maps), size_t(npt) * n_types * sizeof(float)));
+    GridArgs A;
+    A.S = sc->view();
+    A.probe_types = sc->d_probe;
+    A.nslot = n_types;
+    A.maps = sc->d_maps;
+    UD2_HIP(launch_grid_build(A, nullptr));
+    UD2_HIP(hipDeviceSynchronize());
+    return UD2_OK;
+}
+
+int ud2_scene_set_grids(ud2_scene* sc, const int32_t n_h[3], const float origin_h[3], float spacing,
+                        const int32_t* probe_types_h, int32_t n_types, const float* maps_h) {
+    if (!sc || !n_h || !origin_h || !maps_h) return fail(UD2_ERR_ARG, "null argument");
+    if (!(spacing > 0.0f)) return fail(UD2_ERR_ARG, "spacing must be positive");
+    int rc = set_probe_types(sc, probe_types_h, n_types);
+    if (rc) return rc;
+    int64_t npt = 1;
+    for (int d = 0; d < 3; d++) {
+        if (n_h[d] < 2) return fail(UD2_ERR_ARG, "grid needs at least two points per axis");
+        sc->gn[d] = n_h[d];
+        sc->gorigin[d] = origin_h[d];
+        npt *= n_h[d];
+    }
+    sc->gspacing = spacing;
+    if (sc->d_maps) UD2_HIP(hipFree(sc->d_maps));
+    sc->d_maps = nullptr;
+    size_t bytes = size_t(npt) * n_types * sizeof(float);
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&sc->d_maps), bytes));
+    UD2_HIP(hipMemcpy(sc->d_maps, maps_h, bytes, hipMemcpyHostToDevice));
+    return UD2_OK;
+}
+
+int ud2_scene_grid_info(const ud2_scene* sc, int32_t n_h[3], float origin_h[3], float* spacing_h,
+                        int32_t* n_types_h) {
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    for (int d = 0; d < 3; d++) {
+        if (n_h) n_h[d] = sc->gn[d];
+        if (origin_h) origin_h[d] = sc->gorigin[d];
+    }
+    if (spacing_h) *spacing_h = sc->gspacing;
+    if (n_types_h) *n_types_h = int32_t(sc->probe_types.size());
+    return UD2_OK;
+}
+
+int ud2_scene_download_maps(const ud2_scene* sc, float* maps_h) {
+    if (!sc || !maps_h) return fail(UD2_ERR_ARG, "null argument");
+    if (!sc->d_maps) return fail(UD2_ERR_STATE, "scene has no grid maps");
+    size_t n = size_t(sc->gn[0]) * sc->gn[1] * sc->gn[2] * sc->probe_types.size();
+    UD2_HIP(hipMemcpy(maps_h, sc->d_maps, n * sizeof(float), hipMemcpyDeviceToHost));
+    return UD2_OK;
+}
+
+int ud2_scene_destroy(ud2_scene* sc) {
+    if (!sc) return UD2_OK;
+    if (sc->d_rec) hipFree(sc->d_rec);
+    if (sc->d_maps) hipFree(sc->d_maps);
+    if (sc->d_probe) hipFree(sc->d_probe);
+    delete sc;
+    return UD2_OK;
+}
+
+// ------------------------------------------------------------------ ligands
+int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
+    if (!out) return fail(UD2_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!b || b->n_lig <= 0) return fail(UD2_ERR_ARG, "empty ligand batch");
+    if (!b->atom_off_h || !b->tors_off_h || !b->pair_off_h || !b->types_h || !b->xyz_h)
+        return fail(UD2_ERR_ARG, "null ligand array");
+    const int n = b->n_lig;
+    const int nt_all = b->tors_off_h[n];
+    if (nt_all > 0 && (!b->axis_h || !b->dihe0_h || !b->rot_off_h || !b->rot_atoms_h || !b->range_off_h || !b->ranges_h))
+        return fail(UD2_ERR_ARG, "null torsion array");
+    if (b->pair_off_h[n] > 0 && !b->pairs_h) return fail(UD2_ERR_ARG, "null pair array");
+    if (b->atom_off_h[0] != 0 || b->tors_off_h[0] != 0 || b->pair_off_h[0] != 0)
+        return fail(UD2_ERR_ARG, "CSR offsets must start at 0");
+
+    std::vector<DevLig> ligs(n);
+    std::vector<float4> x0(b->atom_off_h[n]);
+    std::vector<int2> axis(nt_all), trange(nt_all);
+    std::vector<float> dihe0(nt_all);
+    std::vector<float2> ranges;
+    std::vector<unsigned long long> mask(size_t(nt_all) * kMaxApl, 0ull);
+    std::vector<uint2> pairs;
+    std::vector<uint8_t> has_type(kNumTypes, 0);
+    for (int l = 0; l < n; l++) {
+        const int a0 = b->atom_off_h[l], na = b->atom_off_h[l + 1] - a0;
+        const int t0 = b->tors_off_h[l], nt = b->tors_off_h[l + 1] - t0;
+        const int p0 = b->pair_off_h[l], np = b->pair_off_h[l + 1] - p0;
+        const std::string tag = "ligand " + std::to_string(l) + ": ";
+        if (na <= 0 || nt < 0 || np < 0) return fail(UD2_ERR_ARG, tag + "negative or empty sizes");
+        if (na > kMaxAtoms) return fail(UD2_ERR_LIMIT, tag + "more than " + std::to_string(kMaxAtoms) + " atoms");
+        if (nt > kMaxTors) return fail(UD2_ERR_LIMIT, tag + "more than " + std::to_string(kMaxTors) + " torsions");
+        DevLig& d = ligs[l];
+        d.natom = na;
+        d.ntors = nt;
+        d.atom_off = a0;
+        d.tors_off = t0;
+        d.pair_off = int32_t(pairs.size());
+        d.apl = (na + kWave - 1) / kWave;
+        d.nheavy = 0;
+        for (int a = 0; a < na; a++) {
+            int t = b->types_h[a0 + a];
+            if (t < 0 || t >= kNumTypes) return fail(UD2_ERR_ARG, tag + "atom type out of range");
+            float4 v;
+            v.x = b->xyz_h[3 * (a0 + a)];
+            v.y = b->xyz_h[3 * (a0 + a) + 1];
+            v.z = b->xyz_h[3 * (a0 + a) + 2];
+            if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z))
+                return fail(UD2_ERR_ARG, tag + "non-finite input coordinate");
+            std::memcpy(&v.w, &t, sizeof(int));
+            x0[a0 + a] = v;
+            if (t != 0) {
+                d.nheavy++;
+                has_type[t] = 1;
+            }
+        }
+        for (int t = 0; t < nt; t++) {
+            const int tg = t0 + t;
+            int from = b->axis_h[2 * tg], to = b->axis_h[2 * tg + 1];
+            if (from < 0 || from >= na || to < 0 || to >= na || from == to)
+                return fail(UD2_ERR_ARG, tag + "torsion axis atom out of range");
+            axis[tg] = make_int2(from, to);
+            dihe0[tg] = b->dihe0_h[tg];
+            const int r0 = b->rot_off_h[tg], r1 = b->rot_off_h[tg + 1];
+            if (r1 < r0) return fail(UD2_ERR_ARG, tag + "bad rotated-atom offsets");
+            for (int k = r0; k < r1; k++) {
+                int a = b->rot_atoms_h[k];
+                if (a < 0 || a >= na) return fail(UD2_ERR_ARG, tag + "rotated atom out of range");
+                mask[size_t(tg) * kMaxApl + a / kWave] |= 1ull << (a % kWave);
+            }
+            const int q0 = b->range_off_h[tg], q1 = b->range_off_h[tg + 1];
+            if (q1 <= q0) return fail(UD2_ERR_ARG, tag + "torsion without an allowed range");
+            trange[tg] = make_int2(int(ranges.size()), q1 - q0);
+            for (int k = q0; k < q1; k++) {
+                float lo = b->ranges_h[2 * k], hi = b->ranges_h[2 * k + 1];
+                if (!(lo <= hi)) return fail(UD2_ERR_ARG, tag + "torsion range with lo > hi");
+                ranges.push_back(make_float2(lo, hi));
+            }
+        }
+        for (int k = 0; k < np; k++) {
+            int i = b->pairs_h[2 * (p0 + k)], j = b->pairs_h[2 * (p0 + k) + 1];
+            if (i < 0 || i >= na || j < 0 || j >= na || i == j) return fail(UD2_ERR_ARG, tag + "pair atom out of range");
+            int ti = b->types_h[a0 + i], tj = b->types_h[a0 + j];
+            if (ti == 0 || tj == 0) continue;  // eval_ef is zero for hydrogen pairs (vina.h:223-225)
+            uint2 rec;
+            rec.x = uint32_t(i) | (uint32_t(j) << 8) | (uint32_t(pair_flags_host(ti, tj)) << 16);
+            float rs = kVdwHost[ti] + kVdwHost[tj];
+            std::memcpy(&rec.y, &rs, sizeof(float));
+            pairs.push_back(rec);
+        }
+        d.npair = int32_t(pairs.size()) - d.pair_off;
+    }
+    ud2_ligset* ls = new ud2_ligset();
+    hipGetDevice(&ls->device);
+    ls->n_lig = n;
+    ls->h_ligs = ligs;
+    ls->has_slot_type = has_type;
+    int rc = UD2_OK;
+    if (!rc) rc = upload(&ls->d_ligs, ligs);
+    if (!rc) rc = upload(&ls->d_x0, x0);
+    if (!rc) rc = upload(&ls->d_axis, axis);
+    if (!rc) rc = upload(&ls->d_dihe0, dihe0);
+    if (!rc) rc = upload(&ls->d_trange, trange);
+    if (!rc) rc = upload(&ls->d_ranges, ranges);
+    if (!rc) rc = upload(&ls->d_mask, mask);
+    if (!rc) rc = upload(&ls->d_pairs, pairs);
+    if (!rc) rc = upload(&ls->d_counter, std::vector<int32_t>(kMaxApl + 1, 0));
+    if (!rc) rc = upload(&ls->d_order, std::vector<int32_t>(n, 0));
+    if (rc) {
+        ud2_ligset_destroy(ls);
+        return rc;
+    }
+    *out = ls;
+    return UD2_OK;
+}
+
+int ud2_ligset_destroy(ud2_ligset* ls) {
+    if (!ls) return UD2_OK;
+    void* ptrs[] = {ls->d_ligs, ls->d_x0, ls->d_axis, ls->d_dihe0, ls->d_trange, ls->d_ranges,
+                    ls->d_mask, ls->d_pairs, ls->d_counter, ls->d_order, ls->d_sel_all};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete ls;
+    return UD2_OK;
+}
+
+int ud2_ligset_pose_layout(const ud2_ligset* ls, int32_t nchain, int64_t* x_floats_h, int64_t* dihe_floats_h,
+                           int64_t* n_pose_h) {
+    if (!ls || nchain <= 0) return fail(UD2_ERR_ARG, "bad arguments");
+    int64_t na = 0, nt = 0;
+    for (auto& l : ls->h_ligs) {
+        na += l.natom;
+        nt += l.ntors;
+    }
+    if (x_floats_h) *x_floats_h = na * nchain * 3;
+    if (dihe_floats_h) *dihe_floats_h = nt * nchain;
+    if (n_pose_h) *n_pose_h = int64_t(ls->n_lig) * nchain;
+    return UD2_OK;
+}
+
+int ud2_init_poses(const ud2_ligset* ls, const ud2_poses* P, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    const int64_t npose = int64_t(ls->n_lig) * P->nchain;
+    if (npose > (int64_t(1) << 30)) return fail(UD2_ERR_LIMIT, "too many poses");
+    std::vector<int32_t> ids(npose);
+    std::iota(ids.begin(), ids.end(), 0);
+    int32_t* d_ids = nullptr;
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&d_ids), ids.size() * sizeof(int32_t)));
+    hipError_t e = hipMemcpy(d_ids, ids.data(), ids.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        rc = run_opt(nullptr, ls, P, d_ids, int32_t(npose), UD2_MODE_DIRECT, 4, 0, 0.0f, 0.0f, nullptr, nullptr,
+                     nullptr, 0.0f, static_cast<hipStream_t>(stream));
+    hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    hipFree(d_ids);
+    if (e != hipSuccess) return fail(UD2_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    return rc;
+}
+
+// ------------------------------------------------------------------ hot path
+int ud2_score_posevecs(const ud2_scene* sc, const ud2_ligset* ls, int32_t lig, int64_t n, const float* pv_d,
+                       float* e_d, float* g_d, int32_t mode, float v_cap, float slope, void* stream) {
+    if (!ls) return fail(UD2_ERR_ARG, "null ligand set");
+    int rc = check_mode(sc, ls, mode);
+    if (rc) return rc;
+    if (lig < 0 || lig >= ls->n_lig) return fail(UD2_ERR_ARG, "ligand index out of range");
+    if (n < 0) return fail(UD2_ERR_ARG, "negative pose count");
+    if (n == 0) return UD2_OK;
+    if (!pv_d || !e_d || !g_d) return fail(UD2_ERR_ARG, "null buffer");
+    const DevLig& d = ls->h_ligs[lig];
+    PoseVecArgs A;
+    A.L = ls->view();
+    A.S = sc->view();
+    A.lig = lig;
+    A.n = n;
+    A.pv = pv_d;
+    A.e = e_d;
+    A.g = g_d;
+    A.v_cap = v_cap;
+    A.slope = slope;
+    A.mode = mode;
+    A.lds_floats_per_wave = lds_fixed_floats(d.apl);
+    const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
+    const int64_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t cap = int64_t(n_cus()) * posevec_blocks_per_cu(d.apl, lds_bytes);
+    const int nb = int(std::max<int64_t>(1, std::min(need, cap)));
+    UD2_HIP(launch_posevec(A, d.apl, nb, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
+int ud2_energy_grad(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, int32_t mode, float v_cap, float slope, float* e_d, float* g_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (!rc) rc = check_mode(sc, ls, mode);
+    if (rc) return rc;
+    return run_opt(sc, ls, P, sel_d, n_sel, mode, 1, 0, v_cap, slope, e_d, g_d, nullptr, 0.0f,
+                   static_cast<hipStream_t>(stream));
+}
+
+int ud2_pose_update(const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel, const float* dir_d,
+                    float alpha, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (n_sel > 0 && !dir_d) return fail(UD2_ERR_ARG, "null direction buffer");
+    return run_opt(nullptr, ls, P, sel_d, n_sel, UD2_MODE_DIRECT, 2, 0, 0.0f, 0.0f, nullptr, nullptr, dir_d, alpha,
+                   static_cast<hipStream_t>(stream));
+}
+
+int ud2_line_search(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, int32_t mode, float v_cap, float slope, float* alpha_d, float* g_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (!rc) rc = check_mode(sc, ls, mode);
+    if (rc) return rc;
+    return run_opt(sc, ls, P, sel_d, n_sel, mode, 3, 0, v_cap, slope, alpha_d, g_d, nullptr, 0.0f,
+                   static_cast<hipStream_t>(stream));
+}
+
+int ud2_optimize(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                 int32_t n_sel, int32_t mode, int32_t max_steps, float v_cap, float slope, float* e_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (!rc) rc = check_mode(sc, ls, mode);
+    if (rc) return rc;
+    if (max_steps < 0) return fail(UD2_ERR_ARG, "negative max_steps");
+    return run_opt(sc, ls, P, sel_d, n_sel, mode, 0, max_steps, v_cap, slope, e_d, nullptr, nullptr, 0.0f,
+                   static_cast<hipStream_t>(stream));
+}
+
+int ud2_mc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_mc_params* prm, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!prm) return fail(UD2_ERR_ARG, "null parameters");
+    rc = check_mode(sc, ls, prm->mode);
+    if (rc) return rc;
+    if (prm->mc_steps < 0 || prm->opt_steps < 0) return fail(UD2_ERR_ARG, "negative step count");
+    if (int64_t(ls->n_lig) * P->nchain > (int64_t(1) << 30)) return fail(UD2_ERR_LIMIT, "too many chains");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    McArgs A;
+    A.L = ls->view();
+    A.S = sc->view();
+    A.P = pose_view(P);
+    A.mode = prm->mode;
+    A.mc_steps = prm->mc_steps;
+    A.opt_steps = prm->opt_steps;
+    A.randomize = prm->randomize;
+    A.seed = prm->seed;
+    A.lig_index_base = prm->lig_index_base;
+    A.stats = reinterpret_cast<unsigned long long*>(prm->stats_d);
+    // Work order: per size class, most expensive ligands first, so the tail
+    // of the persistent launch holds the cheapest chains.
+    std::vector<int32_t> order(ls->n_lig);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int l) {
+        const DevLig& d = ls->h_ligs[l];
+        return double(d.npair + 8 * d.natom) * double(6 + d.ntors);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (ls->h_ligs[a].apl != ls->h_ligs[b].apl) return ls->h_ligs[a].apl > ls->h_ligs[b].apl;
+        return cost(a) > cost(b);
+    });
+    UD2_HIP(hipMemcpyAsync(ls->d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    UD2_HIP(launch_fill_i32(ls->d_counter, 0, kMaxApl + 1, st));
+    int pos = 0;
+    const int cus = n_cus();
+    for (int apl = kMaxApl; apl >= 1; apl--) {
+        int cnt = 0, dmax = 6;
+        for (int k = pos; k < ls->n_lig && ls->h_ligs[order[k]].apl == apl; k++) {
+            cnt++;
+            dmax = std::max(dmax, 6 + int(ls->h_ligs[order[k]].ntors));
+        }
+        if (cnt == 0) continue;
+        A.order = ls->d_order + pos;
+        A.n_order = cnt;
+        A.counter = ls->d_counter + apl;
+        A.hdim_ld = odd_ld(dmax);
+        A.lds_floats_per_wave = lds_fixed_floats(apl) + dmax * A.hdim_ld;
+        const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
+        if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
+        const int64_t items = int64_t(cnt) * P->nchain;
+        const int64_t need = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+        const int64_t cap = int64_t(cus) * mc_blocks_per_cu(apl, lds_bytes);
+        const int nb = int(std::max<int64_t>(1, std::min(need, cap)));
+        UD2_HIP(launch_mc(A, apl, nb, st));
+        pos += cnt;
+    }
+    return UD2_OK;
+}
+
+int ud2_cluster(const ud2_ligset* ls, const ud2_poses* P, float rmsd_limit, int32_t* leaders_d, int32_t* nleaders_d,
+                void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!leaders_d || !nleaders_d) return fail(UD2_ERR_ARG, "null output");
+    if (size_t(2) * P->nchain * sizeof(int) > 64 * 1024) return fail(UD2_ERR_LIMIT, "nchain too large for clustering");
+    ClusterArgs A;
+    A.L = ls->view();
+    A.x = P->x_d;
+    A.e = P->e_d;
+    A.nchain = P->nchain;
+    A.n_lig = ls->n_lig;
+    A.limit = rmsd_limit;
+    A.leaders = leaders_d;
+    A.nleaders = nleaders_d;
+    UD2_HIP(launch_cluster(A, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
+int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, float* out3_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    if (n_sel <= 0) return UD2_OK;
+    if (!sel_d || !out3_d) return fail(UD2_ERR_ARG, "null buffer");
+    std::vector<int32_t> h(n_sel);
+    UD2_HIP(hipMemcpy(h.data(), sel_d, size_t(n_sel) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int apl_max = 1;
+    const int64_t npose = int64_t(ls->n_lig) * P->nchain;
+    for (int32_t p : h) {
+        if (p < 0 || p >= npose) return fail(UD2_ERR_ARG, "pose id out of range: " + std::to_string(p));
+        apl_max = std::max(apl_max, int(ls->h_ligs[p / P->nchain].apl));
+    }
+    ScoreArgs A;
+    A.L = ls->view();
+    A.S = sc->view();
+    A.x = P->x_d;
+    A.nchain = P->nchain;
+    A.sel = sel_d;
+    A.n_sel = n_sel;
+    A.out3 = out3_d;
+    // one launch at the widest class: narrower ligands run with idle register slots
+    UD2_HIP(launch_final_score(A, apl_max, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+"""ctypes binding of libud2.so, the HIP engine (C ABI in include/ud2.h).
+
+The library is built in-tree (``unidock_engine/lib/libud2.so``) by
+``__graft_entry__.build()`` / ``python -m unidock_engine.build``. Loading fails
+loudly when it is missing: the engine has no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libud2.so")
+
+UD2_OK = 0
+MODE_DIRECT = 0
+MODE_GRID = 1
+MAX_ATOMS = 256
+MAX_TORSIONS = 57
+
+
+class Ud2Error(RuntimeError):
+    pass
+
+
+class LigandBatch(C.Structure):
+    _fields_ = [("n_lig", C.c_int32)] + [(n, C.c_void_p) for n in (
+        "atom_off_h", "tors_off_h", "pair_off_h", "types_h", "xyz_h", "axis_h", "dihe0_h", "rot_off_h",
+        "rot_atoms_h", "range_off_h", "ranges_h", "pairs_h")]
+
+
+class Poses(C.Structure):
+    _fields_ = [("x_d", C.c_void_p), ("c_d", C.c_void_p), ("rv_d", C.c_void_p), ("dihe_d", C.c_void_p),
+                ("e_d", C.c_void_p), ("nchain", C.c_int32)]
+
+
+class McParams(C.Structure):
+    _fields_ = [("mc_steps", C.c_int32), ("opt_steps", C.c_int32), ("randomize", C.c_int32),
+                ("seed", C.c_uint32), ("mode", C.c_int32), ("lig_index_base", C.c_int32),
+                ("stats_d", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Ud2Error(f"libud2.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+    sig = {
+        "ud2_version": (C.c_char_p, []),
+        "ud2_last_error": (C.c_char_p, []),
+        "ud2_device_count": (C.c_int, [vp]),
+        "ud2_set_device": (C.c_int, [i32]),
+        "ud2_synchronize": (C.c_int, []),
+        "ud2_scene_create": (C.c_int, [vp, vp, i32, vp, vp, vp]),
+        "ud2_scene_build_grids": (C.c_int, [vp, f32, vp, i32]),
+        "ud2_scene_set_grids": (C.c_int, [vp, vp, vp, f32, vp, i32, vp]),
+        "ud2_scene_grid_info": (C.c_int, [vp, vp, vp, vp, vp]),
+        "ud2_scene_download_maps": (C.c_int, [vp, vp]),
+        "ud2_scene_destroy": (C.c_int, [vp]),
+        "ud2_ligset_create": (C.c_int, [vp, vp]),
+        "ud2_ligset_destroy": (C.c_int, [vp]),
+        "ud2_ligset_pose_layout": (C.c_int, [vp, i32, vp, vp, vp]),
+        "ud2_init_poses": (C.c_int, [vp, vp, vp]),
+        "ud2_score_posevecs": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f32, f32, vp]),
+        "ud2_energy_grad": (C.c_int, [vp, vp, vp, vp, i32, i32, f32, f32, vp, vp, vp]),
+        "ud2_pose_update": (C.c_int, [vp, vp, vp, i32, vp, f32, vp]),
+        "ud2_line_search": (C.c_int, [vp, vp, vp, vp, i32, i32, f32, f32, vp, vp, vp]),
+        "ud2_optimize": (C.c_int, [vp, vp, vp, vp, i32, i32, i32, f32, f32, vp, vp]),
+        "ud2_mc": (C.c_int, [vp, vp, vp, vp, vp]),
+        "ud2_cluster": (C.c_int, [vp, vp, f32, vp, vp, vp]),
+        "ud2_final_score": (C.c_int, [vp, vp, vp, vp, i32, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = (
+    "ud2_version", "ud2_last_error", "ud2_device_count", "ud2_set_device", "ud2_synchronize", "ud2_scene_create",
+    "ud2_scene_build_grids", "ud2_scene_set_grids", "ud2_scene_grid_info", "ud2_scene_download_maps",
+    "ud2_scene_destroy", "ud2_ligset_create", "ud2_ligset_destroy", "ud2_ligset_pose_layout", "ud2_init_poses",
+    "ud2_score_posevecs", "ud2_energy_grad", "ud2_pose_update", "ud2_line_search", "ud2_optimize", "ud2_mc",
+    "ud2_cluster", "ud2_final_score",
+)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != UD2_OK:
+        msg = lib().ud2_last_error().decode(errors="replace")
+        raise Ud2Error(f"{what or 'ud2 call'} failed (code {rc}): {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Scene:
+    """Receptor + box (+ grid maps) on the device."""
+
+    def __init__(self, receptor, box):
+        xyz = np.ascontiguousarray(receptor.xyz, dtype=np.float32).reshape(-1)
+        ty = np.ascontiguousarray(receptor.types, dtype=np.int32)
+        lo = np.ascontiguousarray(box.lo, dtype=np.float32)
+        hi = np.ascontiguousarray(box.hi, dtype=np.float32)
+        self.box = box
+        self.h = C.c_void_p()
+        check(lib().ud2_scene_create(_ptr(xyz), _ptr(ty), int(ty.size), _ptr(lo), _ptr(hi), C.byref(self.h)),
+              "ud2_scene_create")
+
+    def build_grids(self, spacing: float, probe_types) -> None:
+        pt = np.ascontiguousarray(probe_types, dtype=np.int32)
+        check(lib().ud2_scene_build_grids(self.h, float(spacing), _ptr(pt), int(pt.size)), "ud2_scene_build_grids")
+
+    def set_grids(self, n, origin, spacing, probe_types, maps) -> None:
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        org = np.ascontiguousarray(origin, dtype=np.float32)
+        pt = np.ascontiguousarray(probe_types, dtype=np.int32)
+        m = np.ascontiguousarray(maps, dtype=np.float32)
+        check(lib().ud2_scene_set_grids(self.h, _ptr(nn), _ptr(org), float(spacing), _ptr(pt), int(pt.size), _ptr(m)),
+              "ud2_scene_set_grids")
+
+    def grid_info(self):
+        n = np.zeros(3, dtype=np.int32)
+        org = np.zeros(3, dtype=np.float32)
+        sp = C.c_float()
+        nt = C.c_int32()
+        check(lib().ud2_scene_grid_info(self.h, _ptr(n), _ptr(org), C.byref(sp), C.byref(nt)), "ud2_scene_grid_info")
+        return n, org, sp.value, nt.value
+
+    def download_maps(self) -> np.ndarray:
+        n, _, _, nt = self.grid_info()
+        out = np.zeros((nt, n[2], n[1], n[0]), dtype=np.float32)
+        check(lib().ud2_scene_download_maps(self.h, _ptr(out)), "ud2_scene_download_maps")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ud2_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_ligands(ligands) -> dict:
+    """CSR arrays of a ligand list in the layout of ud2_ligand_batch."""
+    n = len(ligands)
+    atom_off = np.zeros(n + 1, dtype=np.int32)
+    tors_off = np.zeros(n + 1, dtype=np.int32)
+    pair_off = np.zeros(n + 1, dtype=np.int32)
+    for i, l in enumerate(ligands):
+        atom_off[i + 1] = atom_off[i] + l.natom
+        tors_off[i + 1] = tors_off[i] + l.ntors
+        pair_off[i + 1] = pair_off[i] + l.npair
+    types = np.concatenate([l.types for l in ligands]).astype(np.int32)
+    xyz = np.concatenate([l.coords.reshape(-1) for l in ligands]).astype(np.float32)
+    axis = np.concatenate([l.axis.reshape(-1) for l in ligands] + [np.zeros(0, np.int32)]).astype(np.int32)
+    dihe0 = np.concatenate([l.dihedrals for l in ligands] + [np.zeros(0, np.float32)]).astype(np.float32)
+    rot_off = [0]
+    rot_atoms = []
+    range_off = [0]
+    ranges = []
+    for l in ligands:
+        for t in range(l.ntors):
+            r = l.rot_atoms[l.rot_start[t]:l.rot_start[t + 1]]
+            rot_atoms.append(r)
+            rot_off.append(rot_off[-1] + len(r))
+            q = l.ranges[l.range_start[t]:l.range_start[t + 1]]
+            ranges.append(q.reshape(-1))
+            range_off.append(range_off[-1] + q.shape[0])
+    pairs = np.concatenate([l.pairs.reshape(-1) for l in ligands] + [np.zeros(0, np.int32)]).astype(np.int32)
+    return dict(
+        atom_off=atom_off, tors_off=tors_off, pair_off=pair_off, types=types, xyz=xyz, axis=axis, dihe0=dihe0,
+        rot_off=np.asarray(rot_off, dtype=np.int32),
+        rot_atoms=(np.concatenate(rot_atoms) if rot_atoms else np.zeros(0)).astype(np.int32),
+        range_off=np.asarray(range_off, dtype=np.int32),
+        ranges=(np.concatenate(ranges) if ranges else np.zeros(0)).astype(np.float32),
+        pairs=pairs,
+    )
+
+
+class LigSet:
+    """A batch of ligand topologies on the device."""
+
+    def __init__(self, ligands):
+        self.ligands = list(ligands)
+        self.arrays = pack_ligands(self.ligands)
+        a = {k: (v if v.size else np.zeros(1, v.dtype)) for k, v in self.arrays.items()}
+        self._keep = a
+        b = LigandBatch(len(self.ligands), *[C.c_void_p(a[k].ctypes.data) for k in (
+            "atom_off", "tors_off", "pair_off", "types", "xyz", "axis", "dihe0", "rot_off", "rot_atoms",
+            "range_off", "ranges", "pairs")])
+        self.h = C.c_void_p()
+        check(lib().ud2_ligset_create(C.byref(b), C.byref(self.h)), "ud2_ligset_create")
+
+    @property
+    def n_lig(self) -> int:
+        return len(self.ligands)
+
+    def layout(self, nchain: int):
+        xf, df, npz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().ud2_ligset_pose_layout(self.h, int(nchain), C.byref(xf), C.byref(df), C.byref(npz)),
+              "ud2_ligset_pose_layout")
+        return xf.value, df.value, npz.value
+
+    def x_offset(self, lig: int, nchain: int, chain: int) -> int:
+        """Float offset of pose (lig, chain) in the x buffer."""
+        return (int(self.arrays["atom_off"][lig]) * nchain + chain * self.ligands[lig].natom) * 3
+
+    def dihe_offset(self, lig: int, nchain: int, chain: int) -> int:
+        return int(self.arrays["tors_off"][lig]) * nchain + chain * self.ligands[lig].ntors
+
+    def close(self):
+        if self.h:
+            lib().ud2_ligset_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_ptr(stream=None):
+    """hipStream_t of a torch stream (None: torch's current stream)."""
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+class PoseBuffers:
+    """Device pose buffers (torch tensors) for nchain poses per ligand of a LigSet."""
+
+    def __init__(self, ligset: LigSet, nchain: int, device="cuda"):
+        import torch
+        self.ligset = ligset
+        self.nchain = int(nchain)
+        xf, df, npz = ligset.layout(nchain)
+        self.n_pose = npz
+        f32 = torch.float32
+        self.x = torch.zeros(max(xf, 1), dtype=f32, device=device)
+        self.c = torch.zeros(max(3 * npz, 1), dtype=f32, device=device)
+        self.rv = torch.zeros(max(3 * npz, 1), dtype=f32, device=device)
+        self.dihe = torch.zeros(max(df, 1), dtype=f32, device=device)
+        self.e = torch.zeros(max(npz, 1), dtype=f32, device=device)
+        self.s = Poses(self.x.data_ptr(), self.c.data_ptr(), self.rv.data_ptr(), self.dihe.data_ptr(),
+                       self.e.data_ptr(), self.nchain)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    def init(self, stream=None):
+        check(lib().ud2_init_poses(self.ligset.h, self.ref, stream_ptr(stream)), "ud2_init_poses")
+
+    def pose(self, lig: int, chain: int) -> dict:
+        """Host copy of one pose."""
+        L = self.ligset.ligands[lig]
+        xo = self.ligset.x_offset(lig, self.nchain, chain)
+        do = self.ligset.dihe_offset(lig, self.nchain, chain)
+        pid = lig * self.nchain + chain
+        return dict(
+            x=self.x[xo:xo + 3 * L.natom].cpu().numpy().copy(),
+            c=self.c[3 * pid:3 * pid + 3].cpu().numpy().copy(),
+            rv=self.rv[3 * pid:3 * pid + 3].cpu().numpy().copy(),
+            dihe=self.dihe[do:do + L.ntors].cpu().numpy().copy(),
+            e=float(self.e[pid].cpu()),
+        )
+
+    def set_pose(self, lig: int, chain: int, x, c, rv, dihe, e=999.0):
+        import torch
+        L = self.ligset.ligands[lig]
+        xo = self.ligset.x_offset(lig, self.nchain, chain)
+        do = self.ligset.dihe_offset(lig, self.nchain, chain)
+        pid = lig * self.nchain + chain
+        dev = self.x.device
+        self.x[xo:xo + 3 * L.natom] = torch.as_tensor(np.asarray(x, np.float32).reshape(-1), device=dev)
+        self.c[3 * pid:3 * pid + 3] = torch.as_tensor(np.asarray(c, np.float32), device=dev)
+        self.rv[3 * pid:3 * pid + 3] = torch.as_tensor(np.asarray(rv, np.float32), device=dev)
+        if L.ntors:
+            self.dihe[do:do + L.ntors] = torch.as_tensor(np.asarray(dihe, np.float32), device=dev)
+        self.e[pid] = float(e)
+
+
+def version() -> str:
+    return lib().ud2_version().decode()
