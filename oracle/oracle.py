@@ -174,6 +174,101 @@ def eval_decomp(d: float, t1: int, t2: int) -> np.ndarray:
     return out
 
 
+def vina_terms(d: float):
+    e = np.zeros(5)
+    f = np.zeros(5)
+    lib().ud2o_vina_terms(C.c_double(d), _p(e), _p(f))
+    return e, f
+
+
+def gaussian(d, off, w):
+    f = C.c_double()
+    L = lib()
+    L.ud2o_gaussian.restype = C.c_double
+    e = L.ud2o_gaussian(C.c_double(d), C.c_double(off), C.c_double(w), C.byref(f))
+    return e, f.value
+
+
+def _dfn(name):
+    fn = getattr(lib(), name)
+    fn.restype = C.c_double
+    return fn
+
+
+def normalize_angle(x):
+    return _dfn("ud2o_normalize_angle")(C.c_double(x))
+
+
+def normalize_angle_2pi(x):
+    return _dfn("ud2o_normalize_angle_2pi")(C.c_double(x))
+
+
+def clamp_by_ranges(a, ranges):
+    r = np.ascontiguousarray(ranges, dtype=np.float32).reshape(-1)
+    return _dfn("ud2o_clamp_by_ranges")(C.c_double(a), _p(r), int(r.size // 2))
+
+
+def radian_in_ranges(ranges, u0, u1):
+    r = np.ascontiguousarray(ranges, dtype=np.float32).reshape(-1)
+    return _dfn("ud2o_radian_in_ranges")(_p(r), int(r.size // 2), C.c_float(u0), C.c_float(u1))
+
+
+def quat_rotate(q, v):
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    vv = np.array(v, dtype=np.float64)
+    lib().ud2o_quat_rotate(_p(qq), _p(vv))
+    return vv
+
+
+def quat_mul(a, b):
+    aa = np.ascontiguousarray(a, dtype=np.float64)
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    o = np.zeros(4)
+    lib().ud2o_quat_mul(_p(aa), _p(bb), _p(o))
+    return o
+
+
+def quat_to_rotvec(q):
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.zeros(3)
+    lib().ud2o_quat_to_rotvec(_p(qq), _p(v))
+    return v
+
+
+def rotvec_to_quat(v):
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    q = np.zeros(4)
+    lib().ud2o_rotvec_to_quat(_p(vv), _p(q))
+    return q
+
+
+def rodrigues(v, axis, th):
+    vv = np.array(v, dtype=np.float64)
+    ax = np.ascontiguousarray(axis, dtype=np.float64)
+    lib().ud2o_rodrigues(_p(vv), _p(ax), C.c_double(th))
+    return vv
+
+
+def drot_dv(v, k):
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    o = np.zeros(9)
+    lib().ud2o_drot_dv(_p(vv), int(k), _p(o))
+    return o.reshape(3, 3)
+
+
+def super_fibonacci(idx, n):
+    v = np.zeros(3)
+    lib().ud2o_super_fibonacci(int(idx), int(n), _p(v))
+    return v
+
+
+def grid_eval(scene, slot, x):
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros(3)
+    e = _dfn("ud2o_grid_eval")(scene.ref, int(slot), _p(xx), _p(g))
+    return e, g
+
+
 def philox(ctr, key) -> np.ndarray:
     c = np.asarray(ctr, dtype=np.uint32)
     k = np.asarray(key, dtype=np.uint32)

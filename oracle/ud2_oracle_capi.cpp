@@ -144,6 +144,34 @@ void ud2o_chain_blocks(uint32_t seed, uint32_t lig, uint32_t chain, int n, uint3
 }
 
 double ud2o_eval_ef(double d, int t1, int t2, double* dedd) { return eval_ef<double>(d, t1, t2, dedd); }
+
+// Unweighted terms and derivatives: gauss1, gauss2, repulsion, hydrophobic, hbond (vina.h:139-192).
+void ud2o_vina_terms(double d, double* e5, double* f5) {
+    e5[0] = gaussian<double>(d, kGauss1Offset, kGauss1Width, &f5[0]);
+    e5[1] = gaussian<double>(d, kGauss2Offset, kGauss2Width, &f5[1]);
+    e5[2] = d < 0 ? d * d : 0.0;
+    f5[2] = d < 0 ? 2 * d : 0.0;
+    e5[3] = slope_interp<double>(kHydLow, kHydHigh, d, &f5[3]);
+    e5[4] = slope_interp<double>(kHbLow, kHbHigh, d, &f5[4]);
+}
+double ud2o_gaussian(double d, double off, double w, double* f) { return gaussian<double>(d, off, w, f); }
+double ud2o_normalize_angle(double x) { return normalize_angle<double>(x); }
+double ud2o_normalize_angle_2pi(double x) { return normalize_angle_2pi<double>(x); }
+double ud2o_clamp_by_ranges(double a, const float* ranges, int n) { return clamp_by_ranges<double>(a, ranges, n); }
+double ud2o_radian_in_ranges(const float* ranges, int n, float u0, float u1) {
+    return radian_in_ranges<double>(ranges, n, u0, u1);
+}
+void ud2o_quat_rotate(const double* q, double* v) { quat_rotate<double>(q, v); }
+void ud2o_quat_mul(const double* a, const double* b, double* o) { quat_mul<double>(a, b, o); }
+void ud2o_quat_to_rotvec(const double* q, double* v) { quat_to_rotvec<double>(q, v); }
+void ud2o_rotvec_to_quat(const double* v, double* q) { rotvec_to_quat<double>(v, q); }
+void ud2o_rodrigues(double* v, const double* axis, double th) { rodrigues<double>(v, axis, th); }
+void ud2o_drot_dv(const double* v, int k, double* out9) { drot_dv<double>(v, k, out9); }
+void ud2o_super_fibonacci(int idx, int n, double* v) { super_fibonacci_rotvec<double>(idx, n, v); }
+double ud2o_grid_eval(const ud2o_scene* s, int slot, const double* x, double* grad) {
+    SceneHolder S(s);
+    return grid_eval<double>(S.grid, slot, x, grad);
+}
 void ud2o_eval_decomp(double d, int t1, int t2, double* out5) { eval_decomp<double>(d, t1, t2, out5); }
 
 float ud2o_rmsd_f32(const float* a, const float* b, const int32_t* types, int natom) {

@@ -87,6 +87,9 @@ class Ligand:
     ff_types: np.ndarray | None = None
     charges: np.ndarray | None = None
     json_obj: dict | None = field(default=None, repr=False)
+    pairs_1213: set | None = field(default=None, repr=False)
+    pairs_14: set | None = field(default=None, repr=False)
+    frags: list | None = field(default=None, repr=False)
 
     @property
     def natom(self) -> int:
@@ -222,7 +225,9 @@ def ligand_from_json(name: str, obj: dict, use_tor_lib: bool = False) -> Ligand:
         rot_lists.append([int(v) for v in tor[3]])
     frags = split_torsions_into_frags(obj.get("root_atoms", []), list(zip(axis, rot_lists)))
     pairs = intra_pairs(n, p1213, p14, frags)
-    return make_ligand(name, coords, types, dihe, axis, rot_lists, range_lists, pairs, ff, q, obj)
+    lig = make_ligand(name, coords, types, dihe, axis, rot_lists, range_lists, pairs, ff, q, obj)
+    lig.pairs_1213, lig.pairs_14, lig.frags = p1213, p14, frags
+    return lig
 
 
 def make_ligand(name, coords, types, dihedrals, axis, rot_lists, range_lists, pairs, ff_types=None, charges=None,
