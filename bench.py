@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=2, help="ligands timed on the CPU oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["batch", "score"], default="batch",
+                    help="batch: docking throughput (headline); score: E+gradient microbenchmark")
+    ap.add_argument("--n-pose", type=int, default=1 << 20)
     return ap.parse_args()
 
 
@@ -74,12 +77,92 @@ def cpu_baseline(ligands, receptor, box, nchain, mc_steps, opt_steps, n_sample, 
     }
 
 
+def score_microbench(args):
+    """E + gradient of 1,048,576 random poses of the 40-atom, 8-torsion ligand (grid inter term,
+    curl cap 10): evals/s and the gather roofline (32 B of map corners per heavy atom per eval)."""
+    import numpy as np
+    import torch
+    from unidock_engine import _native as N
+    from unidock_engine import synthetic as S
+
+    N.check(N.lib().ud2_set_device(0), "ud2_set_device")
+    receptor = S.make_receptor()
+    box = S.docking_box()
+    lig = S.config2_ligand().ligand
+    mode = N.MODE_GRID if args.mode == "grid" else N.MODE_DIRECT
+    scene = N.Scene(receptor, box)
+    if mode == N.MODE_GRID:
+        scene.build_grids(S.GRID_SPACING, [2, 3, 4, 5, 6, 10, 11, 15])
+    ls = N.LigSet([lig])
+    n = args.n_pose
+    pv = torch.as_tensor(S.random_pose_vectors(n, lig.ntors, box, seed=3), device="cuda")
+    e = torch.zeros(n, device="cuda")
+    g = torch.zeros((n, 6 + lig.ntors), device="cuda")
+    st = N.stream_ptr()
+
+    def run():
+        N.check(N.lib().ud2_score_posevecs(scene.h, ls.h, 0, n, C.c_void_p(pv.data_ptr()),
+                                           C.c_void_p(e.data_ptr()), C.c_void_p(g.data_ptr()), mode, 10.0, 1e6, st),
+                "ud2_score_posevecs")
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    evs = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        run()
+        b.record()
+        evs.append((a, b))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    ms = np.asarray([a.elapsed_time(b) for a, b in evs])
+    bytes_per_eval = 32 * lig.n_heavy
+    achieved = n * bytes_per_eval / (ms.mean() / 1000.0) / 1e9
+    out = {
+        "metric": "energy+gradient evals/sec (scoring microbenchmark)",
+        "value": round(n * args.steps / wall, 1),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded random poses)",
+        "config": {"workload": f"{n} random poses of a {lig.natom}-atom, {lig.ntors}-torsion ligand, "
+                               f"{lig.npair} intra pairs, {args.mode} inter term",
+                   "kernel_ms": round(float(ms.mean()), 3)},
+        "roofline": {"kernel": "posevec_kernel<1>", "bound": "hbm", "achieved": round(achieved, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": None, "algorithmic_bytes_per_launch": int(n * bytes_per_eval)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        m = min(n, 1 << 16)
+        osc = O.Scene(receptor, box, "direct")
+        _, _, dt = O.score_posevecs(O.Topo(lig), osc, pv[:m].cpu().numpy(), v_cap=10.0, slope=1e6,
+                                    nthreads=threads, sfx="f32")
+        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{m} of the poses, direct receptor sum, FP32, {threads} threads, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(ligands are independent: see DESIGN.md for how the batch would shard)"}))
         sys.exit(2)
+    if args.workload == "score":
+        return score_microbench(args)
     import numpy as np
     import torch
     from unidock_engine import _native as N

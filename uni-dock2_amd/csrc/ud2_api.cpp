@@ -76,6 +76,7 @@ struct ud2_scene {
     std::vector<int32_t> probe_types;
     int32_t slot_of_type[kNumTypes];
     int32_t* d_probe = nullptr;
+    int32_t* d_slot = nullptr;
 
     SceneDev view() const {
         SceneDev s;
@@ -90,7 +91,7 @@ struct ud2_scene {
         s.maps = d_maps;
         s.gspacing = gspacing;
         s.ginv = gspacing > 0 ? 1.0f / gspacing : 0.0f;
-        for (int t = 0; t < kNumTypes; t++) s.slot_of_type[t] = slot_of_type[t];
+        s.slot_tab = d_slot;
         return s;
     }
 };
@@ -318,9 +319,13 @@ static int set_probe_types(ud2_scene* sc, const int32_t* probe_types_h, int32_t 
         if (sc->slot_of_type[t] >= 0) return fail(UD2_ERR_ARG, "duplicate probe type");
         sc->slot_of_type[t] = k;
     }
-    if (sc->d_probe) hipFree(sc->d_probe);
+    if (sc->d_probe) (void)hipFree(sc->d_probe);
+    if (sc->d_slot) (void)hipFree(sc->d_slot);
     sc->d_probe = nullptr;
-    return upload(&sc->d_probe, sc->probe_types);
+    sc->d_slot = nullptr;
+    int rc = upload(&sc->d_probe, sc->probe_types);
+    if (rc) return rc;
+    return upload(&sc->d_slot, std::vector<int32_t>(sc->slot_of_type, sc->slot_of_type + kNumTypes));
 }
 
 int ud2_scene_build_grids(ud2_scene* sc, float spacing, const int32_t* probe_types_h, int32_t n_types) {
@@ -396,7 +401,8 @@ int ud2_scene_destroy(ud2_scene* sc) {
     if (!sc) return UD2_OK;
     if (sc->d_rec) hipFree(sc->d_rec);
     if (sc->d_maps) hipFree(sc->d_maps);
-    if (sc->d_probe) hipFree(sc->d_probe);
+    if (sc->d_probe) (void)hipFree(sc->d_probe);
+    if (sc->d_slot) (void)hipFree(sc->d_slot);
     delete sc;
     return UD2_OK;
 }
@@ -479,11 +485,25 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
                 ranges.push_back(make_float2(lo, hi));
             }
         }
+        // Heavy pairs in diagonal order (j - i, i): 64 consecutive pairs then hit
+        // distinct atoms on each side, so the kernel's LDS force atomics of one
+        // instruction do not collide.
+        std::vector<std::pair<int, int>> pl;
         for (int k = 0; k < np; k++) {
             int i = b->pairs_h[2 * (p0 + k)], j = b->pairs_h[2 * (p0 + k) + 1];
             if (i < 0 || i >= na || j < 0 || j >= na || i == j) return fail(UD2_ERR_ARG, tag + "pair atom out of range");
+            if (i > j) std::swap(i, j);
             int ti = b->types_h[a0 + i], tj = b->types_h[a0 + j];
             if (ti == 0 || tj == 0) continue;  // eval_ef is zero for hydrogen pairs (vina.h:223-225)
+            pl.emplace_back(i, j);
+        }
+        std::stable_sort(pl.begin(), pl.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& c) {
+            const int da = a.second - a.first, dc = c.second - c.first;
+            return da != dc ? da < dc : a.first < c.first;
+        });
+        for (auto& ij : pl) {
+            const int i = ij.first, j = ij.second;
+            const int ti = b->types_h[a0 + i], tj = b->types_h[a0 + j];
             uint2 rec;
             rec.x = uint32_t(i) | (uint32_t(j) << 8) | (uint32_t(pair_flags_host(ti, tj)) << 16);
             float rs = kVdwHost[ti] + kVdwHost[tj];

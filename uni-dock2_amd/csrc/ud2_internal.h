@@ -44,7 +44,7 @@ struct SceneDev {
     int32_t gn[3];
     float gorigin[3];
     float gspacing, ginv;
-    int32_t slot_of_type[kNumTypes];
+    const int32_t* slot_tab;  // [kNumTypes] map of each XS type (-1 = none), device memory
 };
 
 // Pose buffers of a ligand set with nchain poses per ligand, laid out by

@@ -44,16 +44,28 @@ __device__ __forceinline__ float rdl(float v, int l) {
 }
 __device__ __forceinline__ int rdl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-// Butterfly sum: every lane ends with the same, order-independent bits.
-__device__ __forceinline__ float wave_sum(float v) {
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
-    v += __shfl_xor(v, 8);
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
+
+// Sum over the 64 lanes with DPP row exchanges (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror: each lane of a 16-lane row ends with
+// the row sum, in the same bits) and the four row sums read from lanes 0, 16,
+// 32, 48 and added in a fixed order. The result is uniform (scalar register).
+// Called in wave-uniform control flow only (all lanes active).
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    return ((rdl(v, 0) + rdl(v, 16)) + rdl(v, 32)) + rdl(v, 48);
+}
+
+// Hardware reciprocal / square root / reciprocal square root (about 1 ulp).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // A value all lanes agree on, moved to a scalar register.
 __device__ __forceinline__ float uni(float v) {
@@ -74,26 +86,28 @@ __device__ __forceinline__ void wave_lds_fence() {
 // pair flags precomputed: bit0 both hydrophobic, bit1 hydrogen-bond capable.
 __device__ __forceinline__ float eval_ef(float d, int flags, float* dedd) {
     // gauss1: offset 0, width 0.5
-    float e1 = __expf(-4.0f * d * d);
-    float f1 = -8.0f * d * e1;
+    const float e1 = __expf(-4.0f * d * d);
+    const float f1 = -8.0f * d * e1;
     // gauss2: offset 3, width 2
-    float t2 = (d - 3.0f) * 0.5f;
-    float e2 = __expf(-t2 * t2);
-    float f2 = -t2 * e2;
+    const float t2 = (d - 3.0f) * 0.5f;
+    const float e2 = __expf(-t2 * t2);
+    const float f2 = -t2 * e2;
     float e = -0.035579f * e1 - 0.005156f * e2;
     float f = -0.035579f * f1 - 0.005156f * f2;
-    if (d < 0.0f) {  // repulsion
-        e += 0.840245f * d * d;
-        f += 0.840245f * 2.0f * d;
-    }
-    if (flags & 1) {  // hydrophobic, slope_interp(0.5, 1.5)
-        if (d <= 0.5f) e += -0.035069f;
-        else if (d < 1.5f) { e += -0.035069f * (1.5f - d); f += 0.035069f; }
-    }
-    if (flags & 2) {  // hydrogen bond, slope_interp(-0.7, 0)
-        if (d <= -0.7f) e += -0.587439f;
-        else if (d < 0.0f) { e += -0.587439f * (-d * (1.0f / 0.7f)); f += 0.587439f * (1.0f / 0.7f); }
-    }
+    // repulsion, d < 0
+    const float dn = fminf(d, 0.0f);
+    e += 0.840245f * dn * dn;
+    f += 0.840245f * 2.0f * dn;
+    // hydrophobic, slope_interp(0.5, 1.5)
+    const float h = fminf(fmaxf(1.5f - d, 0.0f), 1.0f);
+    const bool hin = (flags & 1) && d > 0.5f && d < 1.5f;
+    e += (flags & 1) ? -0.035069f * h : 0.0f;
+    f += hin ? 0.035069f : 0.0f;
+    // hydrogen bond, slope_interp(-0.7, 0)
+    const float b = fminf(fmaxf(-d * (1.0f / 0.7f), 0.0f), 1.0f);
+    const bool bin = (flags & 2) && d > -0.7f && d < 0.0f;
+    e += (flags & 2) ? -0.587439f * b : 0.0f;
+    f += bin ? 0.587439f * (1.0f / 0.7f) : 0.0f;
     *dedd = f;
     return e;
 }
@@ -107,7 +121,7 @@ __device__ __forceinline__ int pair_flags(int ti, int tj) {
 
 // normalize_angle (E/src/myutils/mymath.h:38-47).
 __device__ __forceinline__ float normalize_angle(float x) {
-    x = fmodf(x, 2.0f * kPi);
+    x = x - 2.0f * kPi * truncf(x * (1.0f / (2.0f * kPi)));  // fmod(x, 2 pi)
     if (x < -kPi) x += 2.0f * kPi;
     if (x > kPi) x -= 2.0f * kPi;
     return x;
@@ -129,10 +143,10 @@ __device__ __forceinline__ Mat3 quat_to_mat(float4 q) {
 
 // rotvec_to_quaternion (E/src/geometry/quaternion.h:90-107); returns (w, x, y, z) in (.x, .y, .z, .w).
 __device__ __forceinline__ float4 rotvec_to_quat(float v0, float v1, float v2) {
-    const float th = sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+    const float th = fsqrt(v0 * v0 + v1 * v1 + v2 * v2);
     float s, c;
-    sincosf(th * 0.5f, &s, &c);
-    const float k = th > kEps ? s / th : 0.0f;
+    __sincosf(th * 0.5f, &s, &c);  // th is a step-sized angle: the hardware sin/cos is accurate here
+    const float k = th > kEps ? s * frcp(th) : 0.0f;
     return th > kEps ? make_float4(c, v0 * k, v1 * k, v2 * k) : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -144,44 +158,38 @@ __device__ __forceinline__ float3 quat_to_rotvec(float4 q) {
     return make_float3(q.y * k, q.z * k, q.w * k);
 }
 
-// cal_grad_of_rot_over_vec (E/src/geometry/rotation.h:80-144): dR/dv_k for k = 0, 1, 2.
-struct Mat3x3 {
-    float m[3][9];
-};
-
-__device__ __forceinline__ Mat3x3 drot_dv(float v0, float v1, float v2) {
-    Mat3x3 M;
-    float ang = sqrtf(v0 * v0 + v1 * v1 + v2 * v2), th = 0.0f;
-    float vb[3] = {0.0f, 0.0f, 0.0f};
+// Orientation gradient of cal_grad_tile (E/src/cuda/bfgs.cuh:122-190): g_k =
+// <dR/dv_k, sum f (x) R^T w> with dR/dv of cal_grad_of_rot_over_vec
+// (E/src/geometry/rotation.h:80-144) evaluated at v', the rotation vector of the
+// same rotation with its angle in [0, pi] (rotvec_to_axis_angle(norm = true),
+// rotation.h:19-46). Since dR/dv'_k R^T = [J(v') e_k]x with J the left Jacobian of
+// SO(3), g = J(v')^T tau, tau = sum_a (x_a - c) x f_a:
+//   g = tau - a v' x tau + b v' x (v' x tau), a = (1 - cos t)/t^2, b = (t - sin t)/t^3.
+__device__ __forceinline__ float3 orient_grad(float v0, float v1, float v2, float t0, float t1, float t2) {
+    const float ang = fsqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    float th = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
     if (ang > kEps) {
-        th = fmodf(ang, 2.0f * kPi);
-        if (th < 0.0f) th += 2.0f * kPi;
-        if (th > kPi) { th = 2.0f * kPi - th; ang = -ang; }
-        vb[0] = v0 / ang; vb[1] = v1 / ang; vb[2] = v2 / ang;
+        th = ang - 2.0f * kPi * floorf(ang * (1.0f / (2.0f * kPi)));  // normalize_angle_2pi
+        float sg = 1.0f;
+        if (th > kPi) { th = 2.0f * kPi - th; sg = -1.0f; }
+        const float k = sg * th * frcp(ang);
+        w0 = v0 * k; w1 = v1 * k; w2 = v2 * k;
     }
-    float s, c;
-    sincosf(th, &s, &c);
-    float sdt, omc;
-    if (th > kEps) { sdt = s / th; omc = (1.0f - c) / th; }
-    else { float t2 = th * th; sdt = 1.0f - t2 / 6.0f; omc = th * 0.5f - th * t2 / 24.0f; }
-    float K[9] = {0.0f, -vb[2], vb[1], vb[2], 0.0f, -vb[0], -vb[1], vb[0], 0.0f};
-    float K2[9] = {-vb[1] * vb[1] - vb[2] * vb[2], vb[0] * vb[1], vb[0] * vb[2],
-                   vb[0] * vb[1], -vb[0] * vb[0] - vb[2] * vb[2], vb[1] * vb[2],
-                   vb[0] * vb[2], vb[1] * vb[2], -vb[0] * vb[0] - vb[1] * vb[1]};
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float e[3] = {k == 0 ? 1.0f : 0.0f, k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f};
-        float vs[3] = {e[0] - vb[k] * vb[0], e[1] - vb[k] * vb[1], e[2] - vb[k] * vb[2]};
-        float S[9] = {0.0f, -vs[2], vs[1], vs[2], 0.0f, -vs[0], -vs[1], vs[0], 0.0f};
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                float mc = e[r] * vb[q] + vb[r] * e[q] - 2.0f * vb[k] * vb[r] * vb[q];
-                M.m[k][r * 3 + q] = uni(c * vb[k] * K[r * 3 + q] + s * vb[k] * K2[r * 3 + q] + sdt * S[r * 3 + q] + omc * mc);
-            }
+    const float q = th * th;
+    float a, b;
+    if (th < 0.5f) {  // Taylor series; truncation error below 1e-9 at th = 0.5
+        a = 0.5f - q * (1.0f / 24.0f - q * (1.0f / 720.0f - q * (1.0f / 40320.0f)));
+        b = (1.0f / 6.0f) - q * (1.0f / 120.0f - q * (1.0f / 5040.0f - q * (1.0f / 362880.0f)));
+    } else {
+        float s, c;
+        __sincosf(th, &s, &c);  // th in [0.5, pi]
+        const float iq = frcp(q);
+        a = (1.0f - c) * iq;
+        b = (th - s) * iq * frcp(th);
     }
-    return M;
+    const float c0 = w1 * t2 - w2 * t1, c1 = w2 * t0 - w0 * t2, c2 = w0 * t1 - w1 * t0;  // v' x tau
+    const float d0 = w1 * c2 - w2 * c1, d1 = w2 * c0 - w0 * c2, d2 = w0 * c1 - w1 * c0;  // v' x (v' x tau)
+    return make_float3(t0 - a * c0 + b * d0, t1 - a * c1 + b * d1, t2 - a * c2 + b * d2);
 }
 
 // Trilinear value and gradient of map `slot` at p; clamped axes have zero derivative.
@@ -237,6 +245,7 @@ struct Ctx {
     int natom, ntors, npair, nheavy, dim;
     int tors_off, pair_off;
     int type[APL];      // XS type; -1 for lanes beyond natom
+    int slot[APL];      // grid map of the atom's type
     int lx, lf, lo, lh, ld;
     const LigSetDev* L;
     const SceneDev* S;
@@ -262,6 +271,7 @@ __device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const 
     for (int k = 0; k < APL; k++) {
         int a = lane() + kWave * k;
         C.type[k] = a < d.natom ? __float_as_int(L.atom_x0[d.atom_off + a].w) : -1;
+        C.slot[k] = (mode == kGrid && C.type[k] > 0) ? S.slot_tab[C.type[k]] : 0;
     }
 }
 
@@ -358,11 +368,11 @@ __device__ __forceinline__ void apply_torsion(const Ctx<APL>& C, WPose<APL>& P, 
     const int2 ax = C.L->tors_axis[tg];
     const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
     float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
-    const float l = sqrtf(ux * ux + uy * uy + uz * uz);
+    const float l = fsqrt(ux * ux + uy * uy + uz * uz);
     if (l <= kEps) { ux = 1.0f; uy = 0.0f; uz = 0.0f; }
-    else { ux /= l; uy /= l; uz /= l; }
+    else { const float il = frcp(l); ux *= il; uy *= il; uz *= il; }
     float s, c;
-    sincosf(inc, &s, &c);
+    __sincosf(inc, &s, &c);
     ux = uni(ux); uy = uni(uy); uz = uni(uz); s = uni(s); c = uni(c);
     const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
 #pragma unroll
@@ -401,6 +411,7 @@ __device__ __forceinline__ void rigid_move(const Ctx<APL>& C, WPose<APL>& P, flo
 // old centre; rot_vec := alpha * dir[3..5]. dir is lane-distributed.
 template <int APL>
 __device__ __forceinline__ void apply_update(const Ctx<APL>& C, WPose<APL>& P, float dir, float alpha) {
+#pragma unroll 1
     for (int t = 0; t < C.ntors; t++) apply_torsion(C, P, t, rdl(dir, 6 + t) * alpha);
     if (C.ntors > 0) centroid(C, P);
     const float o0 = P.c0, o1 = P.c1, o2 = P.c2;
@@ -440,25 +451,26 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
         const int ti = C.type[k];
         float ea = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
         if (C.mode == kGrid) {
-            ea = grid_eval(S, S.slot_of_type[ti], P.x[k], P.y[k], P.z[k], gx, gy, gz);
+            ea = grid_eval(S, C.slot[k], P.x[k], P.y[k], P.z[k], gx, gy, gz);
         } else {
             const float ri = c_vdw[ti];
+#pragma unroll 1
             for (int j = 0; j < S.nrec; j++) {
                 const float4 r = S.rec[j];
                 const float rx = r.x - P.x[k], ry = r.y - P.y[k], rz = r.z - P.z[k];
                 const float rr = rx * rx + ry * ry + rz * rz;
                 if (rr < kR2Cut) {
                     const int tj = __float_as_int(r.w);
-                    float d = sqrtf(rr), dd;
+                    float d = fsqrt(rr), dd;
                     ea += eval_ef(d - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
                     d = fmaxf(d, kEps);
-                    const float fdr = dd / d;
+                    const float fdr = dd * frcp(d);
                     gx -= fdr * rx; gy -= fdr * ry; gz -= fdr * rz;
                 }
             }
         }
         if (ea > 0.0f) {
-            float s = v_cap / (v_cap + ea);
+            float s = v_cap * frcp(v_cap + ea);
             ea *= s;
             s *= s;
             gx *= s; gy *= s; gz *= s;
@@ -479,6 +491,7 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
     // intra-molecular pairs with the per-pair curl; each pair once
     const uint2* pr = C.L->pairs + C.pair_off;
     const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
+#pragma unroll 1
     for (int p = ln; p < C.npair; p += kWave) {
         const uint2 rec = pr[p];
         const int i = rec.x & 255, j = (rec.x >> 8) & 255, fl = (rec.x >> 16) & 3;
@@ -487,17 +500,16 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
         const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
         const float rr = rx * rx + ry * ry + rz * rz;
         if (rr < kR2Cut) {
-            float d = sqrtf(rr), dd;
+            float d = fsqrt(rr), dd;
             float ep = eval_ef(d - rs, fl, &dd);
             d = fmaxf(d, kEps);
-            const float fdr = dd / d;
+            const float fdr = dd * frcp(d);
             float px = -fdr * rx, py = -fdr * ry, pz = -fdr * rz;
-            if (ep > 0.0f) {
-                float s = v_cap / (v_cap + ep);
-                ep *= s;
-                s *= s;
-                px *= s; py *= s; pz *= s;
-            }
+            // per-pair curl, branch-free
+            float s = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
+            ep *= s;
+            s *= s;
+            px *= s; py *= s; pz *= s;
             e += ep;
             atomicAdd(&s_lds[C.lf + i], px);
             atomicAdd(&s_lds[C.lf + n3 + i], py);
@@ -518,42 +530,35 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
     const float E = uni(wave_sum(e));
     if (!G) return E;
 
-    // gradient: centre, orientation, torsions (cal_grad_tile, bfgs.cuh:69-193)
-    float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, so0 = 0.0f, so1 = 0.0f, so2 = 0.0f;
-    Mat3 R = quat_to_mat(rotvec_to_quat(P.r0, P.r1, P.r2));
-#pragma unroll
-    for (int i = 0; i < 9; i++) R.m[i] = uni(R.m[i]);
-    const Mat3x3 M = drot_dv(P.r0, P.r1, P.r2);
+    // gradient (cal_grad_tile, bfgs.cuh:69-193): centre = sum of heavy-atom gradients;
+    // orientation from the torque about the centre (see orient_grad); torsions below.
+    float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, tqx = 0.0f, tqy = 0.0f, tqz = 0.0f;
 #pragma unroll
     for (int k = 0; k < APL; k++) {
         if (!heavy(C, k)) continue;
         sgx += fx[k]; sgy += fy[k]; sgz += fz[k];
         const float wx = P.x[k] - P.c0, wy = P.y[k] - P.c1, wz = P.z[k] - P.c2;
-        // xbar = R^T w
-        const float bx = R.m[0] * wx + R.m[3] * wy + R.m[6] * wz;
-        const float by = R.m[1] * wx + R.m[4] * wy + R.m[7] * wz;
-        const float bz = R.m[2] * wx + R.m[5] * wy + R.m[8] * wz;
-#define UD2_OG(K) (fx[k] * (M.m[K][0] * bx + M.m[K][1] * by + M.m[K][2] * bz) + \
-                   fy[k] * (M.m[K][3] * bx + M.m[K][4] * by + M.m[K][5] * bz) + \
-                   fz[k] * (M.m[K][6] * bx + M.m[K][7] * by + M.m[K][8] * bz))
-        so0 += UD2_OG(0);
-        so1 += UD2_OG(1);
-        so2 += UD2_OG(2);
-#undef UD2_OG
+        tqx += wy * fz[k] - wz * fy[k];
+        tqy += wz * fx[k] - wx * fz[k];
+        tqz += wx * fy[k] - wy * fx[k];
     }
     float gout = 0.0f, v;
     v = wave_sum(sgx); gout = ln == 0 ? v : gout;
     v = wave_sum(sgy); gout = ln == 1 ? v : gout;
     v = wave_sum(sgz); gout = ln == 2 ? v : gout;
-    v = wave_sum(so0); gout = ln == 3 ? v : gout;
-    v = wave_sum(so1); gout = ln == 4 ? v : gout;
-    v = wave_sum(so2); gout = ln == 5 ? v : gout;
+    {
+        const float3 og = orient_grad(P.r0, P.r1, P.r2, wave_sum(tqx), wave_sum(tqy), wave_sum(tqz));
+        gout = ln == 3 ? og.x : gout;
+        gout = ln == 4 ? og.y : gout;
+        gout = ln == 5 ? og.z : gout;
+    }
+#pragma unroll 1
     for (int t = 0; t < C.ntors; t++) {
         const int tg = C.tors_off + t;
         const int2 ax = C.L->tors_axis[tg];
         const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
         float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
-        const float inv = 1.0f / sqrtf(ux * ux + uy * uy + uz * uz);
+        const float inv = frsq(ux * ux + uy * uy + uz * uz);
         ux = uni(ux * inv); uy = uni(uy * inv); uz = uni(uz * inv);
         const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
         float st = 0.0f;
