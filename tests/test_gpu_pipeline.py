@@ -41,3 +41,63 @@ def test_dock_batch_matches_oracle(gpu, receptor, box):
                                      nthreads=8)
         # same random streams: the best pose of both pipelines lands in the same basin
         assert abs(float(E[0, 1]) - float(oe[0, 1])) < 0.5, (E[:, 1], oe[:, 1])
+
+
+def _sdf_coords(path):
+    lines = open(path).read().splitlines()
+    n = int(lines[3].split()[0])
+    return np.asarray([[float(v) for v in lines[4 + i].split()[:3]] for i in range(n)])
+
+
+@pytest.mark.parametrize("mode", ["grid", "direct"])
+def test_5s8i_redocking_feature(gpu, tmp_path, mode):
+    """The reference's feature test E/test/feature/bin (ud2.yaml, validate.py): 5S8I, seed 121,
+    exhaustiveness 512, 40 MC steps, refine 1, free search mode; the best output pose must be
+    within 2.0 A RMSD (all atoms) of the crystal ligand. Run through the drop-in
+    pipeline.DockingPipeline."""
+    import json
+    import os
+    from unidock_engine.api.python import pipeline
+    here = os.path.dirname(os.path.abspath(__file__))
+    doc = json.load(open(os.path.join(here, "golden", "5S8I_unidock2.json")))
+    pipe = pipeline.DockingPipeline(output_dir=str(tmp_path), center_x=-22.33497980811559,
+                                    center_y=13.31094327905649, center_z=27.36396790165424, size_x=30.0,
+                                    size_y=30.0, size_z=30.0, task="screen", search_mode="free",
+                                    exhaustiveness=512, randomize=True, mc_steps=40, opt_steps=-1, refine_steps=1,
+                                    num_pose=10, rmsd_limit=1.0, energy_range=10.0, seed=121, name_json="5S8I")
+    if mode == "direct":
+        pipe.inter_mode = gpu.MODE_DIRECT
+    pipe.set_receptor(doc["receptor"])
+    pipe.add_ligands({k: v for k, v in doc.items() if k not in ("receptor", "score")})
+    pipe.run()
+    out = json.load(open(os.path.join(str(tmp_path), "5S8I_1.json")))
+    (name, poses), = out.items()
+    assert 1 <= len(poses) <= 10
+    ref = _sdf_coords(os.path.join(here, "golden", "5S8I_ligand.sdf"))
+    rmsds = [float(np.sqrt(np.mean(np.sum((np.asarray(p["coords"]).reshape(-1, 3) - ref) ** 2, axis=1))))
+             for p in poses]
+    e = [p["energy"] for p in poses]
+    assert all(len(v) == 7 for v in e)
+    assert min(rmsds) <= 2.0, rmsds
+
+
+def test_1w1p_pipeline_smoke(gpu, tmp_path):
+    """The reference's pybind smoke test (E/api/python/test/test_pipeline.py:131-157): 1W1P,
+    7,689 receptor atoms, a rigid 21-atom ligand; it must run and write poses."""
+    import json
+    import os
+    from unidock_engine.api.python import pipeline
+    here = os.path.dirname(os.path.abspath(__file__))
+    doc = json.load(open(os.path.join(here, "golden", "1W1P_unidock2.json")))
+    lig = {k: v for k, v in doc.items() if k not in ("receptor", "score")}
+    xyz = np.asarray([a[:3] for v in lig.values() for a in v["atoms"]], float)
+    c = xyz.mean(axis=0)
+    pipe = pipeline.DockingPipeline(output_dir=str(tmp_path), center_x=c[0], center_y=c[1], center_z=c[2],
+                                    size_x=25.0, size_y=25.0, size_z=25.0, search_mode="fast", seed=7)
+    pipe.set_receptor(doc["receptor"])
+    pipe.add_ligands(lig)
+    pipe.run()
+    files = [f for f in os.listdir(str(tmp_path)) if f.endswith(".json")]
+    assert files
+    out = json.load(open(os.path.join(str(tmp_path), files[0])))
+    assert all(len(p) >= 1 for p in out.values())

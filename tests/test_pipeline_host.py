@@ -1,0 +1,88 @@
+"""Host-side logic of the drop-in pipeline (no GPU): presets, grouping, output format."""
+import numpy as np
+import pytest
+
+from unidock_engine.api.python import pipeline as P
+
+
+def test_signature_and_defaults():
+    import inspect
+    sig = inspect.signature(P.DockingPipeline.__init__)
+    kw = [p for p in sig.parameters.values() if p.name != "self"]
+    assert all(p.kind == p.KEYWORD_ONLY for p in kw)
+    names = [p.name for p in kw]
+    # api/python/pipeline.cpp:167-189
+    assert names == ["output_dir", "center_x", "center_y", "center_z", "size_x", "size_y", "size_z", "task",
+                     "search_mode", "exhaustiveness", "randomize", "mc_steps", "opt_steps", "refine_steps",
+                     "num_pose", "rmsd_limit", "energy_range", "seed", "bias", "bias_k", "constraint_docking",
+                     "use_tor_lib", "energy_decomp", "gpu_device_id", "name_json", "max_gpu_mem"]
+    d = {p.name: p.default for p in kw}
+    assert d["search_mode"] == "balance" and d["exhaustiveness"] == 512 and d["mc_steps"] == 20
+    assert d["opt_steps"] == -1 and d["seed"] == 12345 and d["name_json"] == "from_python_obj"
+
+
+def test_search_mode_presets():
+    for mode, (ex, mc) in {"fast": (128, 20), "balance": (256, 30), "detail": (512, 40)}.items():
+        p = dict(search_mode=mode, exhaustiveness=1, mc_steps=1, opt_steps=5)
+        P.apply_search_mode(p)
+        assert (p["exhaustiveness"], p["mc_steps"], p["opt_steps"]) == (ex, mc, -1)
+    p = dict(search_mode="free", exhaustiveness=3, mc_steps=4, opt_steps=5)
+    P.apply_search_mode(p)
+    assert (p["exhaustiveness"], p["mc_steps"], p["opt_steps"]) == (3, 4, 5)
+    with pytest.raises(ValueError):
+        P.apply_search_mode(dict(search_mode="turbo"))
+
+
+def test_task_presets():
+    p = dict(task="score", exhaustiveness=512, num_pose=10)
+    P.apply_task(p)
+    assert p["exhaustiveness"] == 1 and p["mc_steps"] == 0 and p["num_pose"] == 1 and not p["randomize"]
+    p = dict(task="randomize", exhaustiveness=64)
+    P.apply_task(p)
+    assert p["num_pose"] == 64 and p["rmsd_limit"] == 0.0
+    with pytest.raises(ValueError):
+        P.apply_task(dict(task="benchmark"))
+
+
+def test_size_groups():
+    class L:
+        def __init__(self, n, t):
+            self.natom, self.ntors = n, t
+    ligs = [L(30, 4), L(30, 9), L(90, 10), L(200, 3), L(40, 8), L(161, 0)]
+    g = P.group_ligands(ligs)
+    assert g == [[0, 4], [1], [2], [], [3, 5]]
+
+
+@pytest.mark.parametrize("v,s", [(1.5, "1.5"), (2.0, "2.0"), (-7.123456, "-7.123"), (0.0999, "0.099"),
+                                 (1e-7, "0.0"), (float("nan"), "1000000.0"), (3e9, "1000000.0"),
+                                 (-2e7, "-1000000.0"), (np.float32(0.1), "0.1"), (123.4567, "123.456")])
+def test_float_format(v, s):
+    """rapidjson Writer with SetMaxDecimalPlaces(3) (json.cpp:167-169) and safe_val (json.cpp:80-85)."""
+    assert P.fmt_float(v) == s
+
+
+def test_output_document():
+    class R:
+        name = "lig\"1"
+        energies = [np.asarray([-5.0, -6.0, 0.5, -6.5, 0.0, 1.0, 0.0], np.float32)]
+        coords = [np.asarray([1.0, 2.5, -3.25], np.float32)]
+        dihedrals = [np.asarray([], np.float32)]
+    txt = P.dumps_poses([R()])
+    import json
+    doc = json.loads(txt)
+    assert list(doc) == ['lig"1'] and doc['lig"1'][0]["coords"] == [1.0, 2.5, -3.25]
+    assert " " not in txt and txt.endswith("}")
+
+
+def test_write_retries(tmp_path):
+    path = str(tmp_path / "x.json")
+    P.write_json_with_retry(path, '{"a":1}')
+    assert open(path).read() == '{"a":1}'
+    with pytest.raises(OSError):
+        P.write_json_with_retry(str(tmp_path / "missing" / "x.json"), "{}", retries=2, wait=0.01)
+
+
+def test_unsupported_options_fail_loudly():
+    pipe = P.DockingPipeline(center_x=0, center_y=0, center_z=0, size_x=10, size_y=10, size_z=10, bias="pos")
+    with pytest.raises(RuntimeError, match="Core pipeline failed"):
+        pipe.run()

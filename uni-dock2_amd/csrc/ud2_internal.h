@@ -148,7 +148,8 @@ hipError_t launch_final_score(const ScoreArgs& a, int apl, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 int mc_blocks_per_cu(int apl, int lds_bytes_per_block);
 // LDS floats of a wave slice before its Hessian tile.
-__host__ __device__ constexpr int lds_fixed_floats(int apl) { return 7 * kWave * apl + kWave * (3 * apl + 1) + 8; }
+// coordinates + pair forces (7 * 64 * apl) and two parked poses (64 * (3 * apl + 1) + 8 each)
+__host__ __device__ constexpr int lds_fixed_floats(int apl) { return 7 * kWave * apl + 2 * (kWave * (3 * apl + 1) + 8); }
 int posevec_blocks_per_cu(int apl, int lds_bytes_per_block);
 
 }  // namespace ud2

@@ -150,11 +150,29 @@ __device__ __forceinline__ float4 rotvec_to_quat(float v0, float v1, float v2) {
     return th > kEps ? make_float4(c, v0 * k, v1 * k, v2 * k) : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// quaternion_to_rotvec (quaternion.h:115-130).
+// acos on [-1, 1] by Abramowitz & Stegun 4.4.46: acos(x) = sqrt(1 - x) P7(x) for
+// x in [0, 1] (|error| <= 2e-8), acos(-x) = pi - acos(x).
+__device__ __forceinline__ float acos_as(float x) {
+    const float ax = fabsf(x);
+    float p = -0.0012624911f;
+    p = p * ax + 0.0066700901f;
+    p = p * ax - 0.0170881256f;
+    p = p * ax + 0.0308918810f;
+    p = p * ax - 0.0501743046f;
+    p = p * ax + 0.0889789874f;
+    p = p * ax - 0.2145988016f;
+    p = p * ax + 1.5707963050f;
+    const float r = fsqrt(1.0f - ax) * p;
+    return x < 0.0f ? 3.14159265358979f - r : r;
+}
+
+// quaternion_to_rotvec (quaternion.h:115-130) of a unit quaternion: theta =
+// 2 acos(w), s = sin(theta / 2) = sqrt(1 - w^2).
 __device__ __forceinline__ float3 quat_to_rotvec(float4 q) {
-    const float th = 2.0f * acosf(fminf(fmaxf(q.x, -1.0f), 1.0f));
-    const float s = sinf(th * 0.5f);
-    const float k = s > kEps ? th / s : 0.0f;
+    const float w = fminf(fmaxf(q.x, -1.0f), 1.0f);
+    const float th = 2.0f * acos_as(w);
+    const float s = fsqrt(fmaxf(1.0f - w * w, 0.0f));
+    const float k = s > kEps ? th * frcp(s) : 0.0f;
     return make_float3(q.y * k, q.z * k, q.w * k);
 }
 
@@ -302,17 +320,26 @@ __device__ __forceinline__ void centroid(const Ctx<APL>& C, WPose<APL>& P) {
     P.c2 = uni(wave_sum(sz) * inv);
 }
 
+// Pose slots in the wave's LDS slice: slot 0 parks the BFGS start pose, slot 1
+// the accepted MC pose. A slot holds x, y, z per atom and the dihedrals per lane,
+// then c, rot_vec and the energy.
 template <int APL>
-__device__ __forceinline__ void save_pose(const Ctx<APL>& C, const WPose<APL>& P) {
+__device__ __forceinline__ int pose_slot(const Ctx<APL>& C, int slot) {
+    return C.lo + slot * (kWave * (3 * APL + 1) + 8);
+}
+
+template <int APL>
+__device__ __forceinline__ void save_pose(const Ctx<APL>& C, const WPose<APL>& P, int slot = 0) {
     const int ln = lane();
+    const int lo = pose_slot(C, slot);
 #pragma unroll
     for (int k = 0; k < APL; k++) {
-        s_lds[C.lo + (3 * k) * kWave + ln] = P.x[k];
-        s_lds[C.lo + (3 * k + 1) * kWave + ln] = P.y[k];
-        s_lds[C.lo + (3 * k + 2) * kWave + ln] = P.z[k];
+        s_lds[lo + (3 * k) * kWave + ln] = P.x[k];
+        s_lds[lo + (3 * k + 1) * kWave + ln] = P.y[k];
+        s_lds[lo + (3 * k + 2) * kWave + ln] = P.z[k];
     }
-    s_lds[C.lo + 3 * APL * kWave + ln] = P.dihe;
-    const int u = C.lo + (3 * APL + 1) * kWave;
+    s_lds[lo + 3 * APL * kWave + ln] = P.dihe;
+    const int u = lo + (3 * APL + 1) * kWave;
     if (ln == 0) {
         s_lds[u] = P.c0; s_lds[u + 1] = P.c1; s_lds[u + 2] = P.c2;
         s_lds[u + 3] = P.r0; s_lds[u + 4] = P.r1; s_lds[u + 5] = P.r2;
@@ -321,17 +348,18 @@ __device__ __forceinline__ void save_pose(const Ctx<APL>& C, const WPose<APL>& P
 }
 
 template <int APL>
-__device__ __forceinline__ void restore_pose(const Ctx<APL>& C, WPose<APL>& P) {
+__device__ __forceinline__ void restore_pose(const Ctx<APL>& C, WPose<APL>& P, int slot = 0) {
     wave_lds_fence();
     const int ln = lane();
+    const int lo = pose_slot(C, slot);
 #pragma unroll
     for (int k = 0; k < APL; k++) {
-        P.x[k] = s_lds[C.lo + (3 * k) * kWave + ln];
-        P.y[k] = s_lds[C.lo + (3 * k + 1) * kWave + ln];
-        P.z[k] = s_lds[C.lo + (3 * k + 2) * kWave + ln];
+        P.x[k] = s_lds[lo + (3 * k) * kWave + ln];
+        P.y[k] = s_lds[lo + (3 * k + 1) * kWave + ln];
+        P.z[k] = s_lds[lo + (3 * k + 2) * kWave + ln];
     }
-    P.dihe = s_lds[C.lo + 3 * APL * kWave + ln];
-    const int u = C.lo + (3 * APL + 1) * kWave;
+    P.dihe = s_lds[lo + 3 * APL * kWave + ln];
+    const int u = lo + (3 * APL + 1) * kWave;
     P.c0 = uni(s_lds[u]); P.c1 = uni(s_lds[u + 1]); P.c2 = uni(s_lds[u + 2]);
     P.r0 = uni(s_lds[u + 3]); P.r1 = uni(s_lds[u + 4]); P.r2 = uni(s_lds[u + 5]);
     P.e = uni(s_lds[u + 6]);
@@ -424,14 +452,22 @@ __device__ __forceinline__ void apply_update(const Ctx<APL>& C, WPose<APL>& P, f
     rigid_move(C, P, q, o0, o1, o2, P.c0, P.c1, P.c2);
 }
 
-// cal_e_f_tile + cal_grad_tile (bfgs.cuh:69-394). Returns the energy under the
-// curl cap v_cap; with G, g receives the pose gradient (lane k = component k).
-template <int APL, bool G>
-__device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope, float& g) {
+// Per-atom energy gradient (-force) of one evaluation, atom a on lane a % 64.
+template <int APL>
+struct Forces {
+    float fx[APL], fy[APL], fz[APL];
+};
+
+// cal_e_f_tile (bfgs.cuh:221-394): energy under the curl cap v_cap and the
+// per-atom gradient F.
+template <int APL>
+__device__ float energy_forces(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope, Forces<APL>& F) {
     const SceneDev& S = *C.S;
     const int ln = lane();
     const int n3 = kWave * APL;
-    float fx[APL], fy[APL], fz[APL];
+    float* fx = F.fx;
+    float* fy = F.fy;
+    float* fz = F.fz;
     float e = 0.0f;
     C.evals++;
     // coordinates to LDS for the pair loop; zero the pair-force tile
@@ -527,8 +563,17 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
         fy[k] += s_lds[C.lf + n3 + a];
         fz[k] += s_lds[C.lf + 2 * n3 + a];
     }
-    const float E = uni(wave_sum(e));
-    if (!G) return E;
+    return uni(wave_sum(e));
+}
+
+// cal_grad_tile (bfgs.cuh:69-193): the pose gradient from the per-atom gradient,
+// returned lane-distributed (lane k = component k).
+template <int APL>
+__device__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& P, const Forces<APL>& F) {
+    const int ln = lane();
+    const float* fx = F.fx;
+    const float* fy = F.fy;
+    const float* fz = F.fz;
 
     // gradient (cal_grad_tile, bfgs.cuh:69-193): centre = sum of heavy-atom gradients;
     // orientation from the torque about the centre (see orient_grad); torsions below.
@@ -573,8 +618,17 @@ __device__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap
         v = wave_sum(st);
         gout = ln == 6 + t ? v : gout;
     }
-    g = gout;
-    return E;
+    return gout;
+}
+
+// cal_e_grad_tile (bfgs.cuh:397-410).
+template <int APL>
+__device__ __forceinline__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope,
+                                             float& g) {
+    Forces<APL> F;
+    const float e = energy_forces(C, P, v_cap, slope, F);
+    g = pose_grad(C, P, F);
+    return e;
 }
 
 // line_search_tile (bfgs.cuh:623-670).
@@ -583,13 +637,16 @@ __device__ void line_search(const Ctx<APL>& C, const WPose<APL>& x, float g, flo
                             float slope, WPose<APL>& x_new, float& g_new, float& e_out, float& alpha_out) {
     const float pg = uni(wave_sum(p * g));
     float alpha = 1.0f, e_new = 0.0f;
+    Forces<APL> F;
     for (int trial = 0; trial < 10; trial++) {
         x_new = x;
         apply_update(C, x_new, p, alpha);
-        e_new = energy_grad<APL, true>(C, x_new, v_cap, slope, g_new);
+        e_new = energy_forces(C, x_new, v_cap, slope, F);
         if (e_new - e0 < 1e-4f * alpha * pg) break;
         alpha *= 0.5f;
     }
+    // only the gradient of the trial the search ends on is ever used (bfgs.cuh:641-667)
+    g_new = pose_grad(C, x_new, F);
     e_out = e_new;
     alpha_out = alpha;
 }
@@ -603,7 +660,7 @@ __device__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_ca
     const int hrow = C.lh + ln * C.ld;
     const bool row = ln < dim;
     float g;
-    const float E_ori = energy_grad<APL, true>(C, x, v_cap, slope, g);
+    const float E_ori = energy_grad(C, x, v_cap, slope, g);
     float E = E_ori, E1 = 0.0f, alpha = 0.0f;
     if (row)
         for (int j = 0; j < dim; j++) s_lds[hrow + j] = (j == ln) ? 1.0f : 0.0f;
@@ -748,11 +805,15 @@ __device__ void randomize(const Ctx<APL>& C, WPose<APL>& P, Rng& rng, int idx, i
     const float gc0 = (S.box_lo[0] + (S.box_hi[0] - S.box_lo[0]) * u01(b.x)) - P.c0;
     const float gc1 = (S.box_lo[1] + (S.box_hi[1] - S.box_lo[1]) * u01(b.y)) - P.c1;
     const float gc2 = (S.box_lo[2] + (S.box_hi[2] - S.box_lo[2]) * u01(b.z)) - P.c2;
-    // super-Fibonacci sample idx of n (Alexa, CVPR'22)
+    // super-Fibonacci sample idx of n (Alexa, CVPR'22): angles 2 pi s / phi and
+    // 2 pi s / psi, reduced to one turn before the hardware sin / cos
     const float s = float(idx) + 0.5f;
-    const float r = sqrtf(s / float(n)), R = sqrtf(1.0f - s / float(n));
-    const float al = 2.0f * kPi * s / 1.4142135623730951f, be = 2.0f * kPi * s / 1.5337511687552043f;
-    const float3 rv = quat_to_rotvec(make_float4(r * sinf(al), r * cosf(al), R * sinf(be), R * cosf(be)));
+    const float r = fsqrt(s / float(n)), R = fsqrt(1.0f - s / float(n));
+    const float fa = s * (1.0f / 1.4142135623730951f), fb = s * (1.0f / 1.5337511687552043f);
+    float sa, ca, sb, cb;
+    __sincosf(2.0f * kPi * (fa - floorf(fa)), &sa, &ca);
+    __sincosf(2.0f * kPi * (fb - floorf(fb)), &sb, &cb);
+    const float3 rv = quat_to_rotvec(make_float4(r * sa, r * ca, R * sb, R * cb));
     // torsion t draws block (draw + t) on lane t: a range, then a value in it
     float tv = 0.0f;
     if (ln < C.ntors) {
@@ -856,34 +917,37 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void mc_kernel(McArgs A) {
         initial_pose(C, A.L, lig, acc);
         if (A.randomize) randomize(C, acc, rng, chain, nchain);
         if (A.mc_steps == 0) {
-            float gd;
-            acc.e = energy_grad<APL, false>(C, acc, kVCap, kSlope, gd);
+            Forces<APL> F;
+            acc.e = energy_forces(C, acc, kVCap, kSlope, F);
             store_pose(A.L, A.P, lig, chain, acc);
             continue;
         }
         store_pose(A.L, A.P, lig, chain, acc);
+        save_pose(C, acc, 1);  // the accepted pose lives in LDS between steps
         float best = 1e9f;
         C.evals = 0;
         for (int step = 0; step < A.mc_steps; step++) {
-            WPose<APL> cand = acc;
+            WPose<APL> cand;
+            restore_pose(C, cand, 1);
+            const float acc_e = cand.e;
             mutate(C, cand, rng);
             if (A.opt_steps == 0) {
                 float gd;
-                cand.e = energy_grad<APL, true>(C, cand, kVCap, kSlope, gd);
+                cand.e = energy_grad(C, cand, kVCap, kSlope, gd);
             } else {
                 bfgs(C, cand, A.opt_steps, kVCap, kSlope);
             }
             const uint4 b = rng.next();
-            const bool accepted = (cand.e < acc.e) || (u01(b.x) < __expf((acc.e - cand.e) / 1.2f));
+            const bool accepted = (cand.e < acc_e) || (u01(b.x) < __expf((acc_e - cand.e) / 1.2f));
             if (step == 0 || accepted) {
-                acc = cand;
-                if (acc.e < best) {
-                    if (A.opt_steps > 0) bfgs(C, acc, A.opt_steps, kVCapAuth, kSlope);
-                    if (acc.e < best) {
-                        store_pose(A.L, A.P, lig, chain, acc);
-                        best = acc.e;
+                if (cand.e < best) {
+                    if (A.opt_steps > 0) bfgs(C, cand, A.opt_steps, kVCapAuth, kSlope);
+                    if (cand.e < best) {
+                        store_pose(A.L, A.P, lig, chain, cand);
+                        best = cand.e;
                     }
                 }
+                save_pose(C, cand, 1);
             }
         }
         if (A.stats && lane() == 0) {
@@ -924,7 +988,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void opt_kernel(OptArgs A) {
         if (A.out_e && ln == 0) A.out_e[o] = P.e;
     } else if (A.op == 1) {
         float g;
-        const float e = energy_grad<APL, true>(C, P, A.v_cap, A.slope, g);
+        const float e = energy_grad(C, P, A.v_cap, A.slope, g);
         if (A.out_e && ln == 0) A.out_e[o] = e;
         if (A.out_g && ln < C.dim) A.out_g[(size_t)o * kWave + ln] = g;
     } else if (A.op == 2) {
@@ -933,7 +997,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void opt_kernel(OptArgs A) {
         store_pose(A.L, A.P, lig, chain, P);
     } else {
         float g;
-        const float e0 = energy_grad<APL, true>(C, P, A.v_cap, A.slope, g);
+        const float e0 = energy_grad(C, P, A.v_cap, A.slope, g);
         const float p = ln < C.dim ? -g : 0.0f;
         WPose<APL> xn;
         float gn, e1, al;
@@ -972,7 +1036,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void posevec_kernel(PoseVecA
         const float3 rv = quat_to_rotvec(q);
         P.r0 = uni(rv.x); P.r1 = uni(rv.y); P.r2 = uni(rv.z);
         float g;
-        const float e = energy_grad<APL, true>(C, P, A.v_cap, A.slope, g);
+        const float e = energy_grad(C, P, A.v_cap, A.slope, g);
         if (ln == 0) A.e[i] = e;
         if (ln < dimg) A.g[i * dimg + ln] = g;
     }

@@ -1,0 +1,225 @@
+"""Drop-in for Uni-Dock2's ``unidock_engine.api.python.pipeline`` module.
+
+``DockingPipeline`` keeps the reference's keyword-only constructor, defaults and
+methods (unidock/unidock_engine/api/python/pipeline.cpp:56-217): set_receptor(list),
+add_ligands(dict), run(). run() parses the same engine-JSON objects
+(src/format/rapidjson_parser.cpp semantics, see ``unidock_engine.topology``),
+applies the search-mode and task presets (src/screening/core.cpp:65-209), splits
+the ligands into the reference's size groups and memory-bounded batches
+(src/screening/screening.cpp:17-219), docks each batch on the GPU through
+libud2.so and writes ``<output_dir>/<name_json>_<batch_id>.json`` in the
+reference's output format (src/format/json.cpp:88-191). Failures raise
+RuntimeError("Core pipeline failed.") as the reference does (pipeline.cpp:141-143).
+"""
+from __future__ import annotations
+
+import json
+import logging
+import math
+import os
+import time
+
+import numpy as np
+
+from ... import _native as N
+from ...engine import DockEngine, DockParams, heuristic_opt_steps
+from ...topology import Box, XS_H, ligand_from_json, receptor_from_json
+
+log = logging.getLogger("unidock_engine")
+
+# CoreInputDefaults (src/screening/core.h:9-36) with DockParam's values (src/model/model.h:151-168)
+DEFAULTS = dict(
+    output_dir="./res_ud2", task="screen", search_mode="balance", exhaustiveness=512, randomize=True,
+    mc_steps=20, opt_steps=-1, refine_steps=5, num_pose=10, rmsd_limit=1.0, energy_range=10.0, seed=12345,
+    bias="no", bias_k=0.1, constraint_docking=False, use_tor_lib=False, energy_decomp=False, gpu_device_id=0,
+    name_json="from_python_obj", max_gpu_mem=0,
+)
+
+# size groups of run_screening (src/screening/screening.h:19-26): (max atoms, max torsions)
+GROUPS = ((40, 8), (80, 16), (120, 24), (160, 32))
+MAX_BATCH = 1000  # screening.cpp:183
+MAX_SAFE = 1e6    # json.cpp:78-85
+
+
+def apply_search_mode(p: dict) -> None:
+    """apply_search_mode, src/screening/core.cpp:65-84."""
+    mode = p["search_mode"]
+    if mode in ("", "free"):
+        return
+    presets = {"fast": (128, 20), "balance": (256, 30), "detail": (512, 40)}
+    if mode not in presets:
+        raise ValueError(f"Not supported search_mode: {mode} doesn't belong to (fast, balance, detail, free)")
+    p["exhaustiveness"], p["mc_steps"] = presets[mode]
+    p["opt_steps"] = -1
+
+
+def apply_task(p: dict) -> None:
+    """Task presets of core_pipeline, src/screening/core.cpp:165-209."""
+    t = p["task"]
+    if t == "screen":
+        return
+    if t == "score":
+        p.update(randomize=False, exhaustiveness=1, mc_steps=0, opt_steps=0, refine_steps=0, num_pose=1,
+                 energy_range=999.0, rmsd_limit=999.0)
+    elif t == "mc":
+        p.update(randomize=True, opt_steps=0, refine_steps=0)
+    elif t == "randomize":
+        p.update(randomize=True, mc_steps=0, opt_steps=0, refine_steps=0, num_pose=p["exhaustiveness"],
+                 energy_range=1e9, rmsd_limit=0.0)
+    elif t == "optimize":
+        p.update(randomize=False, exhaustiveness=1, mc_steps=0, opt_steps=0, num_pose=1, energy_range=1e9,
+                 rmsd_limit=0.0)
+    else:
+        raise ValueError(f"Not supported task: {t} doesn't belong to (screen, score, mc, randomize, optimize)")
+
+
+def group_ligands(ligands) -> list:
+    """group_flex_mols, src/screening/screening.cpp:17-41: index lists per size group (+ overflow)."""
+    groups = [[] for _ in range(len(GROUPS) + 1)]
+    for i, l in enumerate(ligands):
+        for g, (na, nt) in enumerate(GROUPS):
+            if l.natom <= na and l.ntors <= nt:
+                groups[g].append(i)
+                break
+        else:
+            groups[-1].append(i)
+    return groups
+
+
+def fmt_float(v: float) -> str:
+    """A float as the reference's rapidjson writer prints it: shortest round-trip digits of the
+    double value, at most 3 decimals (extra digits dropped, not rounded), at least one."""
+    v = float(v)
+    if v != v or math.isinf(v):
+        v = MAX_SAFE
+    v = min(max(v, -MAX_SAFE), MAX_SAFE)
+    r = repr(v)
+    if "e" in r or "E" in r:
+        r = f"{v:.20f}".rstrip("0")
+    if "." not in r:
+        r += ".0"
+    ip, fp = r.split(".")
+    fp = fp[:3].rstrip("0") or "0"
+    return f"{ip}.{fp}"
+
+
+def dumps_poses(results) -> str:
+    """write_poses_to_json, src/format/json.cpp:88-191 (compact, no whitespace)."""
+    parts = []
+    for r in results:
+        poses = []
+        for k in range(len(r.energies)):
+            e = ",".join(fmt_float(x) for x in r.energies[k])
+            c = ",".join(fmt_float(x) for x in np.asarray(r.coords[k], np.float32))
+            d = ",".join(fmt_float(x) for x in np.asarray(r.dihedrals[k], np.float32))
+            poses.append('{"energy":[' + e + '],"coords":[' + c + '],"dihedrals":[' + d + "]}")
+        parts.append(json.dumps(r.name) + ":[" + ",".join(poses) + "]")
+    return "{" + ",".join(parts) + "}"
+
+
+def write_json_with_retry(path: str, text: str, retries: int = 5, wait: float = 5.0) -> None:
+    """dump_poses, src/task/DockTask.cpp:285-334: write, fsync, check the last byte is '}',
+    retry with doubling back-off."""
+    for attempt in range(1, retries + 1):
+        try:
+            with open(path, "w") as f:
+                f.write(text)
+                f.flush()
+                os.fsync(f.fileno())
+            with open(path, "rb") as f:
+                f.seek(-1, os.SEEK_END)
+                if f.read(1) != b"}":
+                    raise IOError("JSON file is incomplete.")
+            return
+        except Exception as e:  # noqa: BLE001 - any I/O failure is retried
+            if attempt == retries:
+                raise
+            log.warning("attempt %d to write %s failed: %s; retrying in %.0f s", attempt, path, e, wait)
+            time.sleep(wait)
+            wait *= 2
+
+
+class DockingPipeline:
+    """Uni-Dock2 molecular docking pipeline on an MI355X."""
+
+    def __init__(self, *, output_dir=DEFAULTS["output_dir"], center_x, center_y, center_z, size_x, size_y, size_z,
+                 task=DEFAULTS["task"], search_mode=DEFAULTS["search_mode"],
+                 exhaustiveness=DEFAULTS["exhaustiveness"], randomize=DEFAULTS["randomize"],
+                 mc_steps=DEFAULTS["mc_steps"], opt_steps=DEFAULTS["opt_steps"],
+                 refine_steps=DEFAULTS["refine_steps"], num_pose=DEFAULTS["num_pose"],
+                 rmsd_limit=DEFAULTS["rmsd_limit"], energy_range=DEFAULTS["energy_range"], seed=DEFAULTS["seed"],
+                 bias=DEFAULTS["bias"], bias_k=DEFAULTS["bias_k"], constraint_docking=DEFAULTS["constraint_docking"],
+                 use_tor_lib=DEFAULTS["use_tor_lib"], energy_decomp=DEFAULTS["energy_decomp"],
+                 gpu_device_id=DEFAULTS["gpu_device_id"], name_json=DEFAULTS["name_json"],
+                 max_gpu_mem=DEFAULTS["max_gpu_mem"]):
+        self.box = Box.from_center_size((center_x, center_y, center_z), (size_x, size_y, size_z))
+        self.params = dict(output_dir=str(output_dir), task=task, search_mode=search_mode,
+                           exhaustiveness=int(exhaustiveness), randomize=bool(randomize), mc_steps=int(mc_steps),
+                           opt_steps=int(opt_steps), refine_steps=int(refine_steps), num_pose=int(num_pose),
+                           rmsd_limit=float(rmsd_limit), energy_range=float(energy_range), seed=int(seed),
+                           bias=bias, bias_k=float(bias_k), constraint_docking=bool(constraint_docking),
+                           use_tor_lib=bool(use_tor_lib), energy_decomp=bool(energy_decomp),
+                           gpu_device_id=int(gpu_device_id), name_json=str(name_json), max_gpu_mem=int(max_gpu_mem))
+        self.json_data: dict = {}
+        self.output_files: list = []
+        self.inter_mode = N.MODE_GRID
+
+    def set_receptor(self, receptor_info: list) -> None:
+        self.json_data["receptor"] = receptor_info
+
+    def add_ligands(self, ligands_info: dict) -> None:
+        for k, v in ligands_info.items():
+            self.json_data[k] = v
+
+    def run(self) -> None:
+        try:
+            self._run()
+        except Exception as e:  # the reference logs and returns 1 from core_pipeline (core.cpp:218-222)
+            log.critical("Core pipeline failed: %s", e)
+            raise RuntimeError("Core pipeline failed.") from e
+
+    # ------------------------------------------------------------------
+    def _run(self) -> None:
+        p = dict(self.params)
+        # the reference serialises the cached objects to JSON text and parses them back
+        doc = json.loads(json.dumps(self.json_data))
+        if p["bias"] not in ("no", "pos", "align"):
+            raise ValueError(f"Not supported bias: {p['bias']} doesn't belong to (no, pos, align)")
+        if p["bias"] != "no" or p["constraint_docking"] or p["energy_decomp"]:
+            raise NotImplementedError("bias terms, constraint docking and energy decomposition are not "
+                                      "implemented by this engine yet")
+        apply_search_mode(p)
+        if p["opt_steps"] < 0:
+            p["opt_steps"] = -1
+        apply_task(p)
+        os.makedirs(p["output_dir"], exist_ok=True)
+        N.check(N.lib().ud2_set_device(p["gpu_device_id"]), "ud2_set_device")
+        receptor = receptor_from_json(doc.get("receptor", []), self.box)
+        ligands = [ligand_from_json(k, v, p["use_tor_lib"]) for k, v in doc.items()
+                   if k not in ("score", "receptor")]
+        if not ligands:
+            log.error("No flexible molecules are found")
+            return
+        eng = DockEngine(receptor, self.box, mode=self.inter_mode)
+        eng.ensure_types(ligands)
+        batch_id = 0
+        opt_steps = p["opt_steps"]
+        for group in group_ligands(ligands):
+            for start in range(0, len(group), MAX_BATCH):
+                idx = group[start:start + MAX_BATCH]
+                batch = [ligands[i] for i in idx]
+                batch_id += 1
+                if opt_steps == -1:  # heuristic of the first batch persists (screening.cpp:206-208)
+                    opt_steps = heuristic_opt_steps(max(l.natom for l in batch))
+                prm = DockParams(nchain=p["exhaustiveness"], mc_steps=p["mc_steps"], opt_steps=opt_steps,
+                                 refine_steps=p["refine_steps"], num_pose=p["num_pose"],
+                                 rmsd_limit=p["rmsd_limit"], energy_range=p["energy_range"], seed=p["seed"],
+                                 randomize=p["randomize"], mode=self.inter_mode)
+                t0 = time.time()
+                ls = N.LigSet(batch)
+                results, _ = eng.dock(ls, prm)
+                path = os.path.join(p["output_dir"], f"{p['name_json']}_{batch_id}.json")
+                write_json_with_retry(path, dumps_poses(results))
+                self.output_files.append(path)
+                log.info("Batch %d (%d ligands) running time: %.1f ms", batch_id, len(batch),
+                         (time.time() - t0) * 1e3)
