@@ -155,6 +155,72 @@ int ud2_cluster(const ud2_ligset* ls, const ud2_poses* P, float rmsd_limit, int3
 int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
                     int32_t n_sel, float* out3_d, void* stream);
 
+/* ------------------------------------------------------------- learned proposal
+ * No reference counterpart: the reference's MC tries one random mutation per
+ * step (E/src/cuda/mc.cu:146-256). The learned search draws n_cand mutations
+ * per step, scores them with a small network on the GPU (BF16 MFMA, FP32
+ * accumulation), rescores the top_k with the exact search energy and refines
+ * the best with BFGS; the Metropolis test and the best-pose bookkeeping are
+ * the reference's (mc.cu:340-392). Every kept pose is exact Vina.
+ *
+ * Network: 80 inputs per candidate (pooled per-atom grid-map statistics by
+ * feature class, and move globals), two ReLU layers of 256, one output. The
+ * parameters are FP32 in the canonical order W1[80][256], b1[256],
+ * W2[256][256], b2[256], w3[256], b3, followed by the fixed input
+ * normalisation mu[80], inv_sigma[80]. Inputs are bf16 bit patterns,
+ * [rows][80], rows in groups of 16 candidates of one MC state. */
+#define UD2_MLP_FEATURES 80
+#define UD2_MLP_HIDDEN 256
+#define UD2_MLP_GROUP 16
+#define UD2_MLP_NPARAM 86785
+#define UD2_MLP_NNORM 160
+
+#define UD2_POLICY_PLAIN 0    /* one random mutation per step, no network (the reference's step) */
+#define UD2_POLICY_EXACT 1    /* n_cand mutations, all scored with the exact energy, no network */
+#define UD2_POLICY_LEARNED 2  /* n_cand mutations, the network's top_k scored exactly */
+#define UD2_POLICY_COLLECT 3  /* n_cand mutations, all BFGS-refined: features and labels for training */
+
+typedef struct ud2_mlp ud2_mlp;
+
+typedef struct {
+    float lr, beta1, beta2, eps;
+} ud2_adam;
+
+typedef struct {
+    ud2_mc_params mc;     /* as for ud2_mc; mc_steps >= 1 */
+    int32_t policy;       /* UD2_POLICY_* */
+    int32_t n_cand;       /* mutations per step, 1..16 (ignored by the plain policy) */
+    int32_t top_k;        /* learned policy: candidates rescored exactly, 1..n_cand */
+    int32_t use_graph;    /* capture the rounds of the search into one HIP graph */
+    uint16_t* collect_x_d;  /* collect: [mc_steps][n_pose][16][80] features (bf16) */
+    float* collect_y_d;     /* collect: [mc_steps][n_pose][16] energies after BFGS */
+} ud2_lmc_params;
+
+/* Network on the current device from params_h[UD2_MLP_NPARAM + UD2_MLP_NNORM]. */
+int ud2_mlp_create(const float* params_h, ud2_mlp** out);
+int ud2_mlp_set_params(ud2_mlp* m, const float* params_h);  /* also resets the Adam state */
+int ud2_mlp_get_params(const ud2_mlp* m, float* params_h);
+int ud2_mlp_destroy(ud2_mlp* m);
+/* y_d[16 n_groups]: scores of the rows of groups groups_d[i] (NULL: 0..n_groups-1) of x_d. */
+int ud2_mlp_forward(const ud2_mlp* m, const uint16_t* x_d, const int32_t* groups_d, int32_t n_groups, float* y_d,
+                    void* stream);
+/* grad_d[UD2_MLP_NPARAM] = gradient of sum_r dy_d[r] * y[r] over the same rows. */
+int ud2_mlp_backward(ud2_mlp* m, const uint16_t* x_d, const int32_t* groups_d, int32_t n_groups, const float* dy_d,
+                     float* grad_d, void* stream);
+/* One Adam step on the minibatch of groups groups_d: group-centred squared
+ * error against labels clipped at (group minimum + 6); loss_d (optional, device) = the loss. */
+int ud2_mlp_train_step(ud2_mlp* m, const uint16_t* x_d, const float* label_d, const int32_t* groups_d,
+                       int32_t n_groups, const ud2_adam* opt, float* loss_d, void* stream);
+/* Network inputs of the selected poses (as if reached by no move from an
+ * accepted energy of 0); needs grid maps for every ligand atom type. */
+int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
+                 uint16_t* x_d, void* stream);
+/* Learned-proposal Monte Carlo search; best pose per chain written to P as
+ * ud2_mc does. mlp may be NULL except for the learned policy. Returns after
+ * the search has finished. */
+int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_lmc_params* params,
+            const ud2_mlp* mlp, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

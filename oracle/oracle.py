@@ -468,3 +468,107 @@ def grid_points(receptor, probe_type: int, pts) -> np.ndarray:
     out = np.zeros(p.shape[0], dtype=np.float64)
     lib().ud2o_grid_points(int(xyz.shape[0]), _p(xyz), _p(ty), int(probe_type), int(p.shape[0]), _p(p), _p(out))
     return out
+
+
+# ------------------------------------------------------- learned proposal
+def _declare_learned(L):
+    vp = C.c_void_p
+    L.ud2o_features_f64.argtypes = [vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, vp]
+    L.ud2o_bf16_round.argtypes = [C.c_int64, vp, vp, vp]
+    L.ud2o_mlp_sizes.restype = C.c_int32
+    L.ud2o_mlp_sizes.argtypes = [vp, vp, vp, vp]
+    L.ud2o_mlp_forward.argtypes = [vp, vp, C.c_int64, vp, C.c_int32, vp]
+    L.ud2o_mlp_forward_exact.argtypes = [vp, vp, C.c_int64, vp, vp]
+    L.ud2o_mlp_backward.argtypes = [vp, vp, C.c_int32, vp, vp, C.c_int32, vp]
+    L.ud2o_group_loss.restype = C.c_double
+    L.ud2o_group_loss.argtypes = [C.c_int32, vp, vp, vp]
+    L.ud2o_adam_step.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32]
+
+
+def _llib():
+    L = lib()
+    if not getattr(L, "_learned_declared", False):
+        _declare_learned(L)
+        L._learned_declared = True
+    return L
+
+
+def mlp_sizes():
+    """(n_param, n_feat, n_hidden, n_norm, group)."""
+    a, b, c, d = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    n = _llib().ud2o_mlp_sizes(C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+    return n, a.value, b.value, c.value, d.value
+
+
+def features(lig, scene: "Scene", x, kind: int = -1, magnitude: float = 0.0, e_acc: float = 0.0,
+             moved: float = 0.0) -> np.ndarray:
+    """FP64 network inputs (80) of a pose with coordinates x [natom*3]."""
+    T = Topo(lig)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    out = np.zeros(80, dtype=np.float64)
+    _llib().ud2o_features_f64(T.ref, scene.ref, _p(xx), int(kind), float(magnitude), float(e_acc), float(moved),
+                              _p(out))
+    return out
+
+
+def bf16_round(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.zeros_like(a)
+    _llib().ud2o_bf16_round(a.size, _p(a), _p(out), None)
+    return out
+
+
+def bf16_bits(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    out = np.zeros(a.shape, dtype=np.uint16)
+    _llib().ud2o_bf16_round(a.size, _p(a), None, _p(out))
+    return out
+
+
+def mlp_forward(params, x_bits, round_: bool = True) -> np.ndarray:
+    """params [n_param + n_norm] float32, x_bits [rows, 80] uint16 -> y [rows] float64."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x_bits, dtype=np.uint16).reshape(-1, 80)
+    n = mlp_sizes()[0]
+    y = np.zeros(x.shape[0], dtype=np.float64)
+    _llib().ud2o_mlp_forward(_p(p), _p(p[n:]), x.shape[0], _p(x), int(round_), _p(y))
+    return y
+
+
+def mlp_forward_exact(params, x) -> np.ndarray:
+    """Unrounded FP64 network on FP64 inputs [rows, 80]."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 80)
+    n = mlp_sizes()[0]
+    y = np.zeros(xx.shape[0], dtype=np.float64)
+    _llib().ud2o_mlp_forward_exact(_p(p), _p(p[n:]), xx.shape[0], _p(xx), _p(y))
+    return y
+
+
+def mlp_backward(params, x_bits, dy, round_: bool = True) -> np.ndarray:
+    """Gradient of sum(dy * y) w.r.t. the trainable parameters (float64 [n_param])."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x_bits, dtype=np.uint16).reshape(-1, 80)
+    d = np.ascontiguousarray(dy, dtype=np.float64)
+    n = mlp_sizes()[0]
+    g = np.zeros(n, dtype=np.float64)
+    _llib().ud2o_mlp_backward(_p(p), _p(p[n:]), x.shape[0], _p(x), _p(d), int(round_), _p(g))
+    return g
+
+
+def group_loss(yhat, label):
+    """(loss, d loss / d yhat) of the group-centred clipped squared error."""
+    yh = np.ascontiguousarray(yhat, dtype=np.float64)
+    lb = np.ascontiguousarray(label, dtype=np.float32)
+    d = np.zeros_like(yh)
+    loss = _llib().ud2o_group_loss(yh.size // 16, _p(yh), _p(lb), _p(d))
+    return loss, d
+
+
+def adam_step(params, m, v, grad, lr, beta1, beta2, eps, step):
+    """In-place Adam step on float32 arrays of the trainable parameters."""
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    g = np.ascontiguousarray(grad, dtype=np.float32)
+    _llib().ud2o_adam_step(_p(params), _p(m), _p(v), _p(g), g.size, float(lr), float(beta1), float(beta2),
+                           float(eps), int(step))

@@ -8,6 +8,7 @@
 #include <thread>
 
 #include "ud2_oracle.hpp"
+#include "ud2_oracle_learned.hpp"
 
 using namespace ud2o;
 
@@ -406,5 +407,57 @@ void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_ty
     }
 
 UD2O_BOTH(DEF_NUMERIC)
+
+}  // extern "C"
+
+// ------------------------------------------------------ learned proposal
+extern "C" {
+
+// Features of one pose with coordinates x[natom*3] (FP64), see ud2_oracle_learned.hpp.
+void ud2o_features_f64(const ud2o_ligand* l, const ud2o_scene* s, const double* x, int32_t kind, float magnitude,
+                       float e_acc, float moved, double* out) {
+    Ligand L = to_lig(l);
+    SceneHolder S(s);
+    MoveInfo mv;
+    mv.kind = kind; mv.magnitude = magnitude; mv.e_acc = e_acc; mv.moved = moved;
+    features<double>(L, x, S.scene, mv, out);
+}
+
+void ud2o_bf16_round(int64_t n, const float* in, float* out, uint16_t* bits) {
+    for (int64_t i = 0; i < n; i++) {
+        if (out) out[i] = bf16_round(in[i]);
+        if (bits) bits[i] = bf16_bits(in[i]);
+    }
+}
+
+int32_t ud2o_mlp_sizes(int32_t* n_feat, int32_t* n_hidden, int32_t* n_norm, int32_t* group) {
+    *n_feat = kFeat; *n_hidden = kHidden; *n_norm = kNNorm; *group = kGroup;
+    return kNParam;
+}
+
+// y[r] for nrow rows of bf16 inputs; params[kNParam], norm[kNNorm].
+void ud2o_mlp_forward(const float* params, const float* norm, int64_t nrow, const uint16_t* x, int32_t round,
+                      double* y) {
+    for (int64_t r = 0; r < nrow; r++) y[r] = mlp_forward_row(params, norm, x + r * kFeat, round != 0);
+}
+
+// Forward pass from unrounded FP64 inputs (finite-difference pins of the backward pass).
+void ud2o_mlp_forward_exact(const float* params, const float* norm, int64_t nrow, const double* x, double* y) {
+    for (int64_t r = 0; r < nrow; r++) y[r] = mlp_forward_row(params, norm, nullptr, false, nullptr, x + r * kFeat);
+}
+
+void ud2o_mlp_backward(const float* params, const float* norm, int32_t nrow, const uint16_t* x, const double* dy,
+                       int32_t round, double* grad) {
+    mlp_backward(params, norm, nrow, x, dy, round != 0, grad);
+}
+
+double ud2o_group_loss(int32_t ngroup, const double* yhat, const float* label, double* dyhat) {
+    return group_loss(ngroup, yhat, label, dyhat);
+}
+
+void ud2o_adam_step(float* params, float* m, float* v, const float* grad, int32_t n, float lr, float beta1,
+                    float beta2, float eps, int32_t step) {
+    adam_step(params, m, v, grad, n, lr, beta1, beta2, eps, step);
+}
 
 }  // extern "C"

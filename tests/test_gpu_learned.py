@@ -1,0 +1,298 @@
+"""Learned MC proposal on the GPU vs the CPU oracle, through the C ABI.
+
+Tolerances: features within one bf16 step (2^-7 relative) plus 2e-3 absolute
+of the FP64 oracle rounded to bf16; network outputs within 2e-3 (1 + |y|) of
+the oracle's bf16-emulating FP64 forward pass; weight gradients within 3e-2
+relative Frobenius error and cosine > 0.999 of the oracle's exact FP64
+backward pass (the device rounds dz to bf16 for the MFMA products). Search
+results are exact Vina energies of real poses; the plain policy's best-score
+distribution matches the persistent MC kernel's; graph capture changes no bit.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VP = C.c_void_p
+NP, NF, NH, NN, G = 86785, 80, 256, 160, 16
+
+
+def dptr(t):
+    return VP(t.data_ptr())
+
+
+def rand_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NP + NN, np.float32)
+    p[:NF * NH] = rng.normal(0, math.sqrt(2.0 / NF), NF * NH)
+    p[NF * NH:NF * NH + NH] = rng.normal(0, 0.1, NH)
+    o = NF * NH + NH
+    p[o:o + NH * NH] = rng.normal(0, math.sqrt(2.0 / NH), NH * NH)
+    o += NH * NH
+    p[o:o + NH] = rng.normal(0, 0.1, NH)
+    o += NH
+    p[o:o + NH] = rng.normal(0, math.sqrt(1.0 / NH), NH)
+    p[NP - 1] = 0.3
+    p[NP:NP + NF] = rng.normal(0, 0.5, NF)
+    p[NP + NF:] = rng.uniform(0.5, 2.0, NF)
+    return p
+
+
+@pytest.fixture(scope="module")
+def scene(gpu, receptor, box):
+    from unidock_engine import synthetic as S
+    sc = gpu.Scene(receptor, box)
+    sc.build_grids(S.GRID_SPACING, [2, 3, 4, 5, 6, 10, 11, 15])
+    return sc
+
+
+@pytest.fixture(scope="module")
+def oscene(scene, receptor, box):
+    n, org, sp, nt = scene.grid_info()
+    slot = [-1] * 21
+    for k, t in enumerate([2, 3, 4, 5, 6, 10, 11, 15]):
+        slot[t] = k
+    return O.Scene(receptor, box, "grid", maps=scene.download_maps(), grid_n=n, grid_origin=org, grid_spacing=sp,
+                   slot_of_type=slot)
+
+
+def bf16_to_f32(t):
+    import torch
+    return (t.to(torch.int32) & 0xFFFF).cpu().numpy().astype(np.uint32).__lshift__(16).view(np.float32)
+
+
+def test_features_match_oracle(gpu, scene, oscene, lig_c2, box):
+    import torch
+    from unidock_engine import synthetic as S
+    ls = gpu.LigSet([lig_c2])
+    n = 64
+    pb = gpu.PoseBuffers(ls, n)
+    topo = O.Topo(lig_c2)
+    pvs = S.random_pose_vectors(n, lig_c2.ntors, box, seed=21)
+    # half the poses drawn near the pocket centre, where the maps vary most
+    pvs[: n // 2, :3] = np.asarray(S.POCKET_CENTER) + np.random.default_rng(5).normal(0, 3, (n // 2, 3))
+    poses = [O.pose_from_vec(topo, pv) for pv in pvs]
+    for c, p in enumerate(poses):
+        pb.set_pose(0, c, p.x, p.c, p.rv, p.dihe[:lig_c2.ntors])
+    sel = torch.arange(n, dtype=torch.int32, device="cuda")
+    x = torch.zeros((n, NF), dtype=torch.int16, device="cuda")
+    gpu.check(gpu.lib().ud2_features(scene.h, ls.h, pb.ref, dptr(sel), n, dptr(x), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    got = bf16_to_f32(x).reshape(n, NF)
+    for c in range(n):
+        x32 = pb.pose(0, c)["x"].astype(np.float64)
+        ref = O.bf16_round(O.features(lig_c2, oscene, x32).astype(np.float32))
+        err = np.abs(got[c] - ref)
+        assert np.all(err <= 2.0 ** -7 * np.abs(ref) + 2e-3), (c, np.argmax(err - 2.0 ** -7 * np.abs(ref)),
+                                                                 got[c], ref)
+
+
+def test_mlp_forward_matches_oracle(gpu):
+    import torch
+    p = rand_params(1)
+    m = gpu.Mlp(p)
+    rng = np.random.default_rng(2)
+    ngroup = 37
+    xb = O.bf16_bits(rng.normal(0, 2, (ngroup * G, NF)).astype(np.float32))
+    xd = torch.as_tensor(xb.view(np.int16), device="cuda")
+    y = torch.zeros(ngroup * G, device="cuda")
+    gpu.check(gpu.lib().ud2_mlp_forward(m.h, dptr(xd), None, ngroup, dptr(y), gpu.stream_ptr()))
+    # a gathered subset of groups through the index list
+    groups = torch.as_tensor(np.asarray([5, 0, 36, 5], np.int32), device="cuda")
+    y2 = torch.zeros(4 * G, device="cuda")
+    gpu.check(gpu.lib().ud2_mlp_forward(m.h, dptr(xd), dptr(groups), 4, dptr(y2), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    ref = O.mlp_forward(p, xb)
+    got = y.cpu().numpy()
+    assert np.all(np.abs(got - ref) <= 2e-3 * (1 + np.abs(ref))), np.abs(got - ref).max()
+    g2 = y2.cpu().numpy().reshape(4, G)
+    for k, g in enumerate([5, 0, 36, 5]):
+        assert np.array_equal(g2[k], got[g * G:(g + 1) * G])
+
+
+def _grad_check(got, ref):
+    blocks = [(0, NF * NH), (NF * NH, NF * NH + NH), (NF * NH + NH, NF * NH + NH + NH * NH),
+              (NF * NH + NH + NH * NH, NP - NH - 1), (NP - NH - 1, NP - 1), (NP - 1, NP)]
+    for a, b in blocks:
+        g, r = got[a:b].astype(np.float64), ref[a:b]
+        nr = np.linalg.norm(r)
+        assert np.linalg.norm(g - r) <= 3e-2 * nr + 1e-6, (a, b, np.linalg.norm(g - r) / max(nr, 1e-30))
+        if nr > 1e-6 and b - a > 1:
+            assert g @ r / (np.linalg.norm(g) * nr) > 0.999
+
+
+def test_mlp_backward_matches_oracle(gpu):
+    import torch
+    p = rand_params(3)
+    m = gpu.Mlp(p)
+    rng = np.random.default_rng(4)
+    ngroup = 32
+    xb = O.bf16_bits(rng.normal(0, 2, (ngroup * G, NF)).astype(np.float32))
+    dy = rng.normal(0, 1, ngroup * G).astype(np.float32)
+    xd = torch.as_tensor(xb.view(np.int16), device="cuda")
+    dyd = torch.as_tensor(dy, device="cuda")
+    grad = torch.zeros(NP, device="cuda")
+    gpu.check(gpu.lib().ud2_mlp_backward(m.h, dptr(xd), None, ngroup, dptr(dyd), dptr(grad), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    ref = O.mlp_backward(p, xb, dy.astype(np.float64))
+    _grad_check(grad.cpu().numpy(), ref)
+
+
+def test_train_step_matches_oracle(gpu):
+    """One Adam step: the loss the device reports and the parameters it leaves equal the
+    oracle's (forward, clipped group-centred loss, backward, Adam)."""
+    import torch
+    p = rand_params(5)
+    m = gpu.Mlp(p)
+    rng = np.random.default_rng(6)
+    ngroup = 64
+    xb = O.bf16_bits(rng.normal(0, 2, (ngroup * G, NF)).astype(np.float32))
+    lab = rng.normal(-6, 2, ngroup * G).astype(np.float32)
+    lab[::37] = 1e6
+    xd = torch.as_tensor(xb.view(np.int16), device="cuda")
+    ld = torch.as_tensor(lab, device="cuda")
+    groups = torch.as_tensor(rng.permutation(ngroup)[:48].astype(np.int32), device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    opt = gpu.Adam(1e-3, 0.9, 0.999, 1e-8)
+    gpu.check(gpu.lib().ud2_mlp_train_step(m.h, dptr(xd), dptr(ld), dptr(groups), 48, C.byref(opt), dptr(loss),
+                                           gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    gi = groups.cpu().numpy()
+    rows = (gi[:, None] * G + np.arange(G)[None, :]).reshape(-1)
+    yh = O.mlp_forward(p, xb[rows])
+    l_ref, dy = O.group_loss(yh, lab[rows])
+    assert abs(float(loss.item()) - l_ref) <= 2e-3 * (1 + l_ref)
+    g = O.mlp_backward(p, xb[rows], dy)
+    pr, mm, vv = p[:NP].copy(), np.zeros(NP, np.float32), np.zeros(NP, np.float32)
+    O.adam_step(pr, mm, vv, g.astype(np.float32), 1e-3, 0.9, 0.999, 1e-8, 1)
+    got = m.params()
+    # Adam moves every parameter by about lr * sign(g); compare the moves
+    d_got, d_ref = got[:NP] - p[:NP], pr - p[:NP]
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    assert np.mean(np.sign(d_got[big]) == np.sign(d_ref[big])) > 0.99
+    assert np.abs(d_got - d_ref).max() <= 2.5e-3
+    assert np.array_equal(got[NP:], p[NP:])  # the normalisation is not trained
+
+
+def test_training_fits_a_ranking_target(gpu):
+    """The device training loop learns a target that depends nonlinearly on two
+    features (within-group order), reducing the loss more than tenfold."""
+    import torch
+    rng = np.random.default_rng(7)
+    p = rand_params(8)
+    p[NP:NP + NF] = 0.0
+    p[NP + NF:] = 1.0
+    m = gpu.Mlp(p)
+    ngroup = 2048
+    x = rng.normal(0, 1, (ngroup * G, NF)).astype(np.float32)
+    lab = (np.maximum(x[:, 3], 0.0) * 2.0 - np.abs(x[:, 10])).astype(np.float32)
+    xd = torch.as_tensor(O.bf16_bits(x).view(np.int16), device="cuda")
+    ld = torch.as_tensor(lab, device="cuda")
+    loss = torch.zeros(1, device="cuda")
+    opt = gpu.Adam(2e-3, 0.9, 0.999, 1e-8)
+    losses = []
+    for it in range(300):
+        groups = torch.as_tensor(rng.permutation(ngroup)[:256].astype(np.int32), device="cuda")
+        gpu.check(gpu.lib().ud2_mlp_train_step(m.h, dptr(xd), dptr(ld), dptr(groups), 256, C.byref(opt),
+                                               dptr(loss), gpu.stream_ptr()))
+        if it in (0, 299):
+            torch.cuda.synchronize()
+            losses.append(float(loss.item()))
+    assert losses[1] < losses[0] / 10, losses
+
+
+def _run_lmc(gpu, scene, ls, nchain, policy, seed, steps=6, opt_steps=6, mlp=None, n_cand=16, top_k=4,
+             use_graph=0, collect=None):
+    import torch
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    prm = gpu.LmcParams()
+    prm.mc = gpu.McParams(mc_steps=steps, opt_steps=opt_steps, randomize=1, seed=seed, mode=gpu.MODE_GRID,
+                          lig_index_base=0)
+    prm.policy, prm.n_cand, prm.top_k, prm.use_graph = policy, n_cand, top_k, use_graph
+    if collect is not None:
+        prm.collect_x_d, prm.collect_y_d = collect[0].data_ptr(), collect[1].data_ptr()
+    gpu.check(gpu.lib().ud2_lmc(scene.h, ls.h, pb.ref, C.byref(prm), mlp.h if mlp else None, gpu.stream_ptr()),
+              "ud2_lmc")
+    torch.cuda.synchronize()
+    return pb
+
+
+def test_lmc_plain_matches_mc_kernel_distribution(gpu, scene, lig_c0):
+    """The round-based plain policy is the reference's MC step: its best-score distribution
+    over 32 seeds matches the persistent MC kernel's (KS p > 0.01, medians within 0.25)."""
+    import torch
+    from scipy.stats import ks_2samp
+    ls = gpu.LigSet([lig_c0])
+    nchain = 16
+    a, b = [], []
+    for seed in range(32):
+        pb = _run_lmc(gpu, scene, ls, nchain, gpu.POLICY_PLAIN, seed, steps=8, opt_steps=8)
+        a.append(pb.e[:nchain].cpu().numpy().min())
+        pm = gpu.PoseBuffers(ls, nchain)
+        pm.init()
+        prm = gpu.McParams(mc_steps=8, opt_steps=8, randomize=1, seed=1000 + seed, mode=gpu.MODE_GRID,
+                           lig_index_base=0)
+        gpu.check(gpu.lib().ud2_mc(scene.h, ls.h, pm.ref, C.byref(prm), gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        b.append(pm.e[:nchain].cpu().numpy().min())
+    a, b = np.asarray(a), np.asarray(b)
+    assert np.isfinite(a).all()
+    assert ks_2samp(a, b).pvalue > 0.01, (np.sort(a), np.sort(b))
+    assert abs(np.median(a) - np.median(b)) < 0.25
+
+
+@pytest.mark.parametrize("policy", ["exact", "learned", "collect"])
+def test_lmc_policies_keep_exact_poses(gpu, scene, lig_c0, lig_c2, policy):
+    """Every policy returns real poses (bond lengths kept) whose stored energy is the exact
+    search energy (V_CAP 1000, slope 1e6) of that pose; the collect policy's labels and
+    features are finite."""
+    import torch
+    ls = gpu.LigSet([lig_c0, lig_c2])
+    nchain = 8
+    pol = {"exact": gpu.POLICY_EXACT, "learned": gpu.POLICY_LEARNED, "collect": gpu.POLICY_COLLECT}[policy]
+    mlp = gpu.Mlp(rand_params(9)) if policy == "learned" else None
+    collect = None
+    steps = 4
+    if policy == "collect":
+        npose = 2 * nchain
+        collect = (torch.zeros(steps * npose * G * NF, dtype=torch.int16, device="cuda"),
+                   torch.zeros(steps * npose * G, dtype=torch.float32, device="cuda"))
+    pb = _run_lmc(gpu, scene, ls, nchain, pol, 3, steps=steps, opt_steps=6, mlp=mlp, collect=collect)
+    e = pb.e[:2 * nchain].cpu().numpy()
+    assert np.isfinite(e).all() and (e < 1e8).all()
+    sel = torch.arange(2 * nchain, dtype=torch.int32, device="cuda")
+    e2 = torch.zeros(2 * nchain, device="cuda")
+    g2 = torch.zeros((2 * nchain, 64), device="cuda")
+    gpu.check(gpu.lib().ud2_energy_grad(scene.h, ls.h, pb.ref, dptr(sel), 2 * nchain, gpu.MODE_GRID, 1000.0, 1e6,
+                                        dptr(e2), dptr(g2), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    assert np.abs(e2.cpu().numpy() - e).max() <= 1e-4 * max(1.0, np.abs(e).max())
+    for li, lig in enumerate([lig_c0, lig_c2]):
+        for c in range(nchain):
+            x = pb.pose(li, c)["x"].reshape(-1, 3)
+            for i, j in lig.axis:
+                d0 = np.linalg.norm(lig.coords[i] - lig.coords[j])
+                assert abs(np.linalg.norm(x[i] - x[j]) - d0) < 1e-3
+    if policy == "collect":
+        lab = collect[1].cpu().numpy()
+        feats = bf16_to_f32(collect[0])
+        assert np.isfinite(lab).all() and np.isfinite(feats).all()
+        assert (lab < 1e8).mean() > 0.9
+
+
+def test_lmc_graph_capture_changes_nothing(gpu, scene, lig_c0, lig_c2):
+    """The same search captured in one HIP graph and launched eagerly gives identical bits."""
+    ls = gpu.LigSet([lig_c0, lig_c2])
+    mlp = gpu.Mlp(rand_params(10))
+    res = []
+    for g in (0, 1):
+        pb = _run_lmc(gpu, scene, ls, 8, gpu.POLICY_LEARNED, 11, steps=3, opt_steps=4, mlp=mlp, use_graph=g)
+        res.append((pb.x.cpu().numpy(), pb.e.cpu().numpy(), pb.dihe.cpu().numpy()))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)

@@ -755,3 +755,430 @@ int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- learned proposal
+struct ud2_mlp {
+    int device = 0;
+    float* d_params = nullptr;  // canonical FP32 parameters, then mu and inv_sigma
+    float* d_m = nullptr;       // Adam moments
+    float* d_v = nullptr;
+    float* d_grad = nullptr;
+    uint16_t* d_w1t = nullptr;
+    uint16_t* d_w2t = nullptr;
+    uint16_t* d_w2 = nullptr;
+    float* d_w3r = nullptr;
+    int32_t step = 0;
+    // training scratch, sized for cap_groups groups
+    int32_t cap_groups = 0;
+    float* d_yhat = nullptr;
+    float* d_dy = nullptr;
+    uint16_t* d_fm = nullptr;   // xn, a1, dz1, dz2 feature-major
+    float* d_part = nullptr;
+    float* d_gpart = nullptr;   // split-K partials of the weight-gradient GEMMs
+    float* d_lparts = nullptr;
+
+    MlpDev view() const {
+        MlpDev w;
+        w.w1t = d_w1t;
+        w.w2t = d_w2t;
+        w.w2 = d_w2;
+        w.b1 = d_params + kOffB1;
+        w.b2 = d_params + kOffB2;
+        w.w3 = d_w3r;
+        w.b3 = d_params + kOffB3;
+        w.mu = d_params + kMlpNParam;
+        w.inv_sigma = d_params + kMlpNParam + kFeat;
+        return w;
+    }
+};
+
+namespace {
+
+constexpr int kGemmSplits = 16;
+
+void free_train(ud2_mlp* m) {
+    for (void* p : {(void*)m->d_yhat, (void*)m->d_dy, (void*)m->d_fm, (void*)m->d_part, (void*)m->d_gpart,
+                    (void*)m->d_lparts})
+        if (p) (void)hipFree(p);
+    m->d_yhat = m->d_dy = m->d_part = m->d_gpart = m->d_lparts = nullptr;
+    m->d_fm = nullptr;
+    m->cap_groups = 0;
+}
+
+int ensure_train(ud2_mlp* m, int32_t n_groups) {
+    if (n_groups <= m->cap_groups) return UD2_OK;
+    free_train(m);
+    const size_t rows = size_t(n_groups) * kGroup;
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_yhat), rows * sizeof(float)));
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_dy), rows * sizeof(float)));
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_fm), rows * size_t(kFeat + 3 * kHidden) * sizeof(uint16_t)));
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_part), size_t(n_groups) * kPartStride * sizeof(float)));
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_gpart), size_t(kGemmSplits) * kHidden * kHidden * sizeof(float)));
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&m->d_lparts), size_t(n_groups) * sizeof(float)));
+    m->cap_groups = n_groups;
+    return UD2_OK;
+}
+
+int check_mlp_inputs(const uint16_t* x_d, int32_t n_groups) {
+    if (n_groups <= 0) return fail(UD2_ERR_ARG, "n_groups must be positive");
+    if (!x_d) return fail(UD2_ERR_ARG, "null feature buffer");
+    if (reinterpret_cast<uintptr_t>(x_d) % 8) return fail(UD2_ERR_ARG, "feature buffer must be 8-byte aligned");
+    return UD2_OK;
+}
+
+// Backward pass and gradient reductions of the minibatch whose d loss / d y_hat is in dy.
+int mlp_grads(ud2_mlp* m, const uint16_t* x_d, const int32_t* groups_d, int32_t n_groups, const float* dy,
+              hipStream_t st) {
+    int rc = ensure_train(m, n_groups);
+    if (rc) return rc;
+    const size_t rows = size_t(n_groups) * kGroup;
+    MlpTrain T;
+    T.x = x_d;
+    T.groups = groups_d;
+    T.n_groups = n_groups;
+    T.dy = dy;
+    T.xn_fm = m->d_fm;
+    T.a1_fm = T.xn_fm + rows * kFeat;
+    T.dz1_fm = T.a1_fm + rows * kHidden;
+    T.dz2_fm = T.dz1_fm + rows * kHidden;
+    T.part = m->d_part;
+    UD2_HIP(launch_mlp_backward(m->view(), T, st));
+    int splits = kGemmSplits;
+    while (splits > 1 && n_groups % splits) splits >>= 1;
+    // dW1[f][h] = sum_r xn[r][f] dz1[r][h]; dW2[h][o] = sum_r a1[r][h] dz2[r][o]
+    UD2_HIP(launch_gemm_nt(T.xn_fm, T.dz1_fm, kFeat, kHidden, int(rows), splits, m->d_gpart, m->d_grad + kOffW1, st));
+    UD2_HIP(launch_gemm_nt(T.a1_fm, T.dz2_fm, kHidden, kHidden, int(rows), splits, m->d_gpart, m->d_grad + kOffW2, st));
+    UD2_HIP(launch_mlp_reduce_parts(m->d_part, n_groups, m->d_grad, st));
+    return UD2_OK;
+}
+
+}  // namespace
+
+int ud2_mlp_create(const float* params_h, ud2_mlp** out) {
+    if (!out) return fail(UD2_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!params_h) return fail(UD2_ERR_ARG, "null parameters");
+    ud2_mlp* m = new ud2_mlp();
+    hipGetDevice(&m->device);
+    const size_t np = kMlpNParam;
+    int rc = UD2_OK;
+    auto al = [&](void** p, size_t bytes) {
+        if (rc == UD2_OK && hipMalloc(p, bytes) != hipSuccess) rc = fail(UD2_ERR_HIP, "hipMalloc failed");
+    };
+    al(reinterpret_cast<void**>(&m->d_params), (np + kMlpNNorm) * sizeof(float));
+    al(reinterpret_cast<void**>(&m->d_m), np * sizeof(float));
+    al(reinterpret_cast<void**>(&m->d_v), np * sizeof(float));
+    al(reinterpret_cast<void**>(&m->d_grad), np * sizeof(float));
+    al(reinterpret_cast<void**>(&m->d_w1t), size_t(kFeat) * kHidden * sizeof(uint16_t));
+    al(reinterpret_cast<void**>(&m->d_w2t), size_t(kHidden) * kHidden * sizeof(uint16_t));
+    al(reinterpret_cast<void**>(&m->d_w2), size_t(kHidden) * kHidden * sizeof(uint16_t));
+    al(reinterpret_cast<void**>(&m->d_w3r), size_t(kHidden) * sizeof(float));
+    if (rc == UD2_OK) rc = ud2_mlp_set_params(m, params_h);
+    if (rc != UD2_OK) {
+        std::string keep = g_err;
+        ud2_mlp_destroy(m);
+        g_err = keep;
+        return rc;
+    }
+    *out = m;
+    return UD2_OK;
+}
+
+int ud2_mlp_set_params(ud2_mlp* m, const float* params_h) {
+    if (!m || !params_h) return fail(UD2_ERR_ARG, "null argument");
+    for (int i = 0; i < kMlpNParam + kMlpNNorm; i++)
+        if (!std::isfinite(params_h[i])) return fail(UD2_ERR_ARG, "non-finite network parameter " + std::to_string(i));
+    UD2_HIP(hipMemcpy(m->d_params, params_h, size_t(kMlpNParam + kMlpNNorm) * sizeof(float), hipMemcpyHostToDevice));
+    UD2_HIP(hipMemset(m->d_m, 0, size_t(kMlpNParam) * sizeof(float)));
+    UD2_HIP(hipMemset(m->d_v, 0, size_t(kMlpNParam) * sizeof(float)));
+    m->step = 0;
+    UD2_HIP(launch_mlp_pack(m->d_params, m->d_w1t, m->d_w2t, m->d_w2, m->d_w3r, nullptr));
+    UD2_HIP(hipDeviceSynchronize());
+    return UD2_OK;
+}
+
+int ud2_mlp_get_params(const ud2_mlp* m, float* params_h) {
+    if (!m || !params_h) return fail(UD2_ERR_ARG, "null argument");
+    UD2_HIP(hipDeviceSynchronize());
+    UD2_HIP(hipMemcpy(params_h, m->d_params, size_t(kMlpNParam + kMlpNNorm) * sizeof(float), hipMemcpyDeviceToHost));
+    return UD2_OK;
+}
+
+int ud2_mlp_destroy(ud2_mlp* m) {
+    if (!m) return UD2_OK;
+    free_train(m);
+    for (void* p : {(void*)m->d_params, (void*)m->d_m, (void*)m->d_v, (void*)m->d_grad, (void*)m->d_w1t,
+                    (void*)m->d_w2t, (void*)m->d_w2, (void*)m->d_w3r})
+        if (p) (void)hipFree(p);
+    delete m;
+    return UD2_OK;
+}
+
+int ud2_mlp_forward(const ud2_mlp* m, const uint16_t* x_d, const int32_t* groups_d, int32_t n_groups, float* y_d,
+                    void* stream) {
+    if (!m || !y_d) return fail(UD2_ERR_ARG, "null argument");
+    int rc = check_mlp_inputs(x_d, n_groups);
+    if (rc) return rc;
+    UD2_HIP(launch_mlp_forward(m->view(), x_d, groups_d, n_groups, y_d, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
+int ud2_mlp_backward(ud2_mlp* m, const uint16_t* x_d, const int32_t* groups_d, int32_t n_groups, const float* dy_d,
+                     float* grad_d, void* stream) {
+    if (!m || !dy_d || !grad_d) return fail(UD2_ERR_ARG, "null argument");
+    int rc = check_mlp_inputs(x_d, n_groups);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = mlp_grads(m, x_d, groups_d, n_groups, dy_d, st);
+    if (rc) return rc;
+    UD2_HIP(hipMemcpyAsync(grad_d, m->d_grad, size_t(kMlpNParam) * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return UD2_OK;
+}
+
+int ud2_mlp_train_step(ud2_mlp* m, const uint16_t* x_d, const float* label_d, const int32_t* groups_d,
+                       int32_t n_groups, const ud2_adam* opt, float* loss_d, void* stream) {
+    if (!m || !label_d || !opt) return fail(UD2_ERR_ARG, "null argument");
+    int rc = check_mlp_inputs(x_d, n_groups);
+    if (rc) return rc;
+    if (!(opt->lr > 0.0f) || !(opt->beta1 >= 0.0f && opt->beta1 < 1.0f) || !(opt->beta2 >= 0.0f && opt->beta2 < 1.0f) ||
+        !(opt->eps > 0.0f))
+        return fail(UD2_ERR_ARG, "invalid Adam settings");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = ensure_train(m, n_groups);
+    if (rc) return rc;
+    UD2_HIP(launch_mlp_forward(m->view(), x_d, groups_d, n_groups, m->d_yhat, st));
+    float* loss = loss_d ? loss_d : m->d_lparts;  // without a loss output the partials' slot 0 takes the sum
+    UD2_HIP(launch_group_loss(m->d_yhat, label_d, groups_d, n_groups, m->d_dy, m->d_lparts, loss, st));
+    rc = mlp_grads(m, x_d, groups_d, n_groups, m->d_dy, st);
+    if (rc) return rc;
+    m->step++;
+    const float c1 = 1.0f - std::pow(opt->beta1, float(m->step));
+    const float c2 = 1.0f - std::pow(opt->beta2, float(m->step));
+    UD2_HIP(launch_adam(m->d_params, m->d_m, m->d_v, m->d_grad, kMlpNParam, opt->lr, opt->beta1, opt->beta2, opt->eps,
+                        c1, c2, st));
+    UD2_HIP(launch_mlp_pack(m->d_params, m->d_w1t, m->d_w2t, m->d_w2, m->d_w3r, st));
+    return UD2_OK;
+}
+
+int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
+                 uint16_t* x_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    rc = check_mode(sc, ls, UD2_MODE_GRID);
+    if (rc) return rc;
+    if (n_sel <= 0) return UD2_OK;
+    if (!sel_d || !x_d) return fail(UD2_ERR_ARG, "null buffer");
+    std::vector<int32_t> h(n_sel);
+    UD2_HIP(hipMemcpy(h.data(), sel_d, size_t(n_sel) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int64_t npose = int64_t(ls->n_lig) * P->nchain;
+    int apl = 1;
+    for (int32_t p : h) {
+        if (p < 0 || p >= npose) return fail(UD2_ERR_ARG, "pose id out of range: " + std::to_string(p));
+        apl = std::max(apl, int(ls->h_ligs[p / P->nchain].apl));
+    }
+    FeatArgs A;
+    A.L = ls->view();
+    A.S = sc->view();
+    A.P = pose_view(P);
+    A.sel = sel_d;
+    A.n_sel = n_sel;
+    A.x = x_d;
+    A.lds_floats_per_wave = lds_prop_floats(apl);
+    UD2_HIP(launch_features(A, apl, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
+// Learned-proposal MC: state buffers of the chains, per-class work orders and
+// one launch sequence per round, optionally captured into a HIP graph.
+int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_lmc_params* prm,
+            const ud2_mlp* mlp, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!prm) return fail(UD2_ERR_ARG, "null parameters");
+    const ud2_mc_params& mc = prm->mc;
+    rc = check_mode(sc, ls, mc.mode);
+    if (rc) return rc;
+    if (mc.mc_steps < 1 || mc.opt_steps < 0) return fail(UD2_ERR_ARG, "mc_steps must be >= 1, opt_steps >= 0");
+    const int policy = prm->policy;
+    if (policy < UD2_POLICY_PLAIN || policy > UD2_POLICY_COLLECT) return fail(UD2_ERR_ARG, "unknown policy");
+    const int n_cand = policy == UD2_POLICY_PLAIN ? 1 : prm->n_cand;
+    if (n_cand < 1 || n_cand > kMaxCand) return fail(UD2_ERR_ARG, "n_cand must be in 1..16");
+    if (policy == UD2_POLICY_LEARNED && (prm->top_k < 1 || prm->top_k > n_cand))
+        return fail(UD2_ERR_ARG, "top_k must be in 1..n_cand");
+    if (policy == UD2_POLICY_LEARNED && !mlp) return fail(UD2_ERR_ARG, "learned policy without a network");
+    const bool feats = policy == UD2_POLICY_LEARNED || policy == UD2_POLICY_COLLECT;
+    if (feats) {
+        rc = check_mode(sc, ls, UD2_MODE_GRID);  // the features read the grid maps
+        if (rc) return rc;
+    }
+    if (policy == UD2_POLICY_COLLECT && (!prm->collect_x_d || !prm->collect_y_d))
+        return fail(UD2_ERR_ARG, "collect policy without output buffers");
+    const int64_t npose = int64_t(ls->n_lig) * P->nchain;
+    if (npose > (int64_t(1) << 26)) return fail(UD2_ERR_LIMIT, "too many chains");
+    hipStream_t caller = static_cast<hipStream_t>(stream);
+
+    // chain state: accepted pose, Philox positions, candidate features / scores
+    int64_t nx = 0, nd = 0, np = 0;
+    rc = ud2_ligset_pose_layout(ls, P->nchain, &nx, &nd, &np);
+    if (rc) return rc;
+    std::vector<void*> owned;
+    auto dev = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
+        owned.push_back(p);
+        return p;
+    };
+    struct Cleanup {
+        std::vector<void*>* v;
+        ~Cleanup() { for (void* p : *v) (void)hipFree(p); }
+    } cleanup{&owned};
+    PoseBufs acc;
+    acc.nchain = P->nchain;
+    acc.x = static_cast<float*>(dev(size_t(nx) * sizeof(float)));
+    acc.dihe = static_cast<float*>(dev(size_t(nd) * sizeof(float)));
+    acc.c = static_cast<float*>(dev(size_t(np) * 3 * sizeof(float)));
+    acc.rv = static_cast<float*>(dev(size_t(np) * 3 * sizeof(float)));
+    acc.e = static_cast<float*>(dev(size_t(np) * sizeof(float)));
+    uint32_t* draw = static_cast<uint32_t*>(dev(size_t(np) * sizeof(uint32_t)));
+    uint32_t* cand_draw = static_cast<uint32_t*>(dev(size_t(np) * kGroup * sizeof(uint32_t)));
+    uint16_t* xs = (feats && policy != UD2_POLICY_COLLECT)
+                       ? static_cast<uint16_t*>(dev(size_t(np) * kGroup * kFeat * sizeof(uint16_t)))
+                       : nullptr;
+    float* pred = policy == UD2_POLICY_LEARNED ? static_cast<float*>(dev(size_t(np) * kGroup * sizeof(float))) : nullptr;
+    std::vector<int32_t> order(ls->n_lig);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int l) {
+        const DevLig& d = ls->h_ligs[l];
+        return double(d.npair + 8 * d.natom) * double(6 + d.ntors);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (ls->h_ligs[a].apl != ls->h_ligs[b].apl) return ls->h_ligs[a].apl > ls->h_ligs[b].apl;
+        return cost(a) > cost(b);
+    });
+    // one queue counter per (round, kernel, class), all cleared once up front
+    const int steps = mc.mc_steps;
+    const int n_counters = (1 + 2 * steps) * (kMaxApl + 1);
+    int32_t* counters = static_cast<int32_t*>(dev(size_t(n_counters) * sizeof(int32_t)));
+    int32_t* d_order = static_cast<int32_t*>(dev(order.size() * sizeof(int32_t)));
+    if (!acc.x || !acc.dihe || !acc.c || !acc.rv || !acc.e || !draw || !cand_draw || !counters || !d_order ||
+        (feats && policy != UD2_POLICY_COLLECT && !xs) || (policy == UD2_POLICY_LEARNED && !pred))
+        return fail(UD2_ERR_HIP, "hipMalloc of the chain state failed");
+    UD2_HIP(hipMemcpy(d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+
+    struct Cls {
+        int apl, pos, cnt, dmax;
+    };
+    std::vector<Cls> classes;
+    for (int pos = 0; pos < ls->n_lig;) {
+        Cls c{int(ls->h_ligs[order[pos]].apl), pos, 0, 6};
+        while (pos < ls->n_lig && int(ls->h_ligs[order[pos]].apl) == c.apl) {
+            c.dmax = std::max(c.dmax, 6 + int(ls->h_ligs[order[pos]].ntors));
+            c.cnt++;
+            pos++;
+        }
+        classes.push_back(c);
+    }
+    const int cus = n_cus();
+    LmcArgs A;
+    A.L = ls->view();
+    A.S = sc->view();
+    A.best = pose_view(P);
+    A.acc = acc;
+    A.mode = mc.mode;
+    A.opt_steps = mc.opt_steps;
+    A.randomize = mc.randomize;
+    A.seed = mc.seed;
+    A.lig_index_base = mc.lig_index_base;
+    A.policy = policy;
+    A.n_cand = n_cand;
+    A.top_k = prm->top_k;
+    A.draw = draw;
+    A.cand_draw = cand_draw;
+    A.pred = pred;
+    A.label = nullptr;
+    A.stats = reinterpret_cast<unsigned long long*>(mc.stats_d);
+
+    // launch sequence; counters are indexed (slot * (kMaxApl + 1) + apl)
+    auto enqueue = [&](hipStream_t st) -> int {
+        UD2_HIP(launch_fill_i32(counters, 0, n_counters, st));
+        auto run_kernel = [&](int kernel, int slot) -> int {
+            for (const Cls& c : classes) {
+                LmcArgs a = A;
+                a.order = d_order + c.pos;
+                a.n_order = c.cnt;
+                a.counter = counters + slot * (kMaxApl + 1) + c.apl;
+                a.hdim_ld = odd_ld(c.dmax);
+                a.lds_floats_per_wave = kernel == 1 ? lds_prop_floats(c.apl)
+                                                    : lds_fixed_floats(c.apl) + c.dmax * a.hdim_ld;
+                const int lds_bytes = a.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
+                if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
+                const int64_t items = int64_t(c.cnt) * P->nchain;
+                const int64_t need = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+                const int64_t cap = int64_t(cus) * lmc_blocks_per_cu(kernel, c.apl, lds_bytes);
+                const int nb = int(std::max<int64_t>(1, std::min(need, cap)));
+                hipError_t e = kernel == 0 ? launch_lmc_init(a, c.apl, nb, st)
+                             : kernel == 1 ? launch_lmc_propose(a, c.apl, nb, st)
+                                           : launch_lmc_step(a, c.apl, nb, st);
+                if (e != hipSuccess) return fail(UD2_ERR_HIP, std::string("lmc launch: ") + hipGetErrorString(e));
+            }
+            return UD2_OK;
+        };
+        int r = run_kernel(0, 0);
+        if (r) return r;
+        for (int s = 0; s < steps; s++) {
+            A.step = s;
+            if (policy == UD2_POLICY_COLLECT) {
+                A.x = prm->collect_x_d + size_t(s) * size_t(np) * kGroup * kFeat;
+                A.label = prm->collect_y_d + size_t(s) * size_t(np) * kGroup;
+            } else {
+                A.x = xs;
+            }
+            if (feats) {
+                r = run_kernel(1, 1 + 2 * s);
+                if (r) return r;
+            }
+            if (policy == UD2_POLICY_LEARNED)
+                UD2_HIP(launch_mlp_forward(mlp->view(), xs, nullptr, np, pred, st));
+            r = run_kernel(2, 2 + 2 * s);
+            if (r) return r;
+        }
+        return UD2_OK;
+    };
+
+    if (!prm->use_graph) {
+        rc = enqueue(caller);
+        if (rc) return rc;
+        // the chain state is freed on return: finish the search first
+        UD2_HIP(hipStreamSynchronize(caller));
+        return UD2_OK;
+    }
+    // graph: captured on a private stream (default priority) ordered after the caller's stream
+    hipStream_t cs = nullptr;
+    UD2_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipEvent_t ev = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto done = [&](int code) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev) (void)hipEventDestroy(ev);
+        if (cs) (void)hipStreamDestroy(cs);
+        return code;
+    };
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return done(fail(UD2_ERR_HIP, "event"));
+    if (hipEventRecord(ev, caller) != hipSuccess || hipStreamWaitEvent(cs, ev, 0) != hipSuccess)
+        return done(fail(UD2_ERR_HIP, "stream ordering"));
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return done(fail(UD2_ERR_HIP, "hipStreamBeginCapture failed"));
+    rc = enqueue(cs);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(cs, &g);
+    graph = g;
+    if (rc) return done(rc);
+    if (ec != hipSuccess) return done(fail(UD2_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec)));
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        return done(fail(UD2_ERR_HIP, "hipGraphInstantiate failed"));
+    if (hipGraphLaunch(exec, cs) != hipSuccess) return done(fail(UD2_ERR_HIP, "hipGraphLaunch failed"));
+    if (hipStreamSynchronize(cs) != hipSuccess) return done(fail(UD2_ERR_HIP, "graph execution failed"));
+    return done(UD2_OK);
+}

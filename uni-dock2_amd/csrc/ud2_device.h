@@ -1,0 +1,923 @@
+// Device-side building blocks of the docking hot path for gfx950 (CDNA4, wave64),
+// shared by the kernel translation units (ud2_kernels.hip, ud2_learned.hip).
+//
+// Execution model: one docking chain (or one scored pose) per wavefront. Atom
+// a lives on lane a % 64, register slot a / 64 (APL = atoms per lane, a
+// template parameter), so coordinates and per-atom gradients stay in VGPRs;
+// the dihedral vector and every BFGS vector (gradient, direction, y, -Hy) are
+// one VGPR each with component k on lane k; the inverse Hessian is a per-wave
+// LDS tile whose row i only lane i reads and writes. Energies, torques and
+// dot products are wave reductions. Intra-molecular pairs are spread over the
+// lanes and scatter their forces into a per-wave LDS force tile with LDS
+// atomics. MFMA is not used: the physics is gather- and VALU-bound.
+//
+// The algorithm follows the reference's device code (E/ =
+// unidock/unidock_engine/): cal_e_f_tile / cal_grad_tile (E/src/cuda/bfgs.cuh:
+// 69-394), the torsion and rigid pose updates (bfgs.cuh:443-618), line search
+// and BFGS (bfgs.cuh:623-829), mutation / randomisation / MC (E/src/cuda/mc.cu:
+// 18-398), RMSD clustering (E/src/cuda/cluster.cu:16-135) and final scoring
+// (E/src/score/score.cpp:12-108).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ud2_internal.h"
+
+namespace ud2 {
+
+// ------------------------------------------------------------- constants
+// vdW radii (E/src/constants/constants.h:21-43) and per-type flags:
+// bit0 hydrophobic, bit1 acceptor, bit2 donor (E/src/score/vina.h:49-65).
+static __constant__ float c_vdw[kNumTypes] = {1.1f, 1.92f, 1.9f, 1.9f, 1.8f, 1.8f, 1.8f, 1.8f, 1.7f, 1.7f, 1.7f,
+                                       1.7f, 1.5f, 2.2f, 2.1f, 2.0f, 1.8f, 2.0f, 2.2f, 2.3f, 1.2f};
+static __constant__ int c_tflags[kNumTypes] = {0, 0, 1, 0, 0, 4, 2, 6, 0, 4, 2, 6, 1, 0, 0, 0, 1, 1, 1, 0, 4};
+
+constexpr float kPi = 3.1415927f;               // E/src/myutils/common.h:45
+constexpr float kEps = 1.1920928955078125e-7f;  // FLT_EPSILON (common.h:13)
+constexpr float kR2Cut = 64.0f;                 // E/src/score/vina.h:130
+constexpr float kVCap = 10.0f, kVCapAuth = 1000.0f, kSlope = 1e6f;  // constants.h:16-19
+constexpr int kTypeH = 0;
+
+// ------------------------------------------------------------ wave utils
+__device__ __forceinline__ int lane() { return __lane_id(); }
+
+__device__ __forceinline__ float rdl(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int rdl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 64 lanes with DPP row exchanges (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror: each lane of a 16-lane row ends with
+// the row sum, in the same bits) and the four row sums read from lanes 0, 16,
+// 32, 48 and added in a fixed order. The result is uniform (scalar register).
+// Called in wave-uniform control flow only (all lanes active).
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    return ((rdl(v, 0) + rdl(v, 16)) + rdl(v, 32)) + rdl(v, 48);
+}
+
+// Hardware reciprocal / square root / reciprocal square root (about 1 ulp).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// A value all lanes agree on, moved to a scalar register.
+__device__ __forceinline__ float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Dynamic LDS of a launch; every wave owns a slice of it.
+extern __shared__ float s_lds[];
+
+// Orders this wave's LDS traffic (one wave owns its LDS slice; no block barrier).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// --------------------------------------------------------------- scoring
+// Vina::eval_ef (E/src/score/vina.h:216-245) for a heavy-atom pair with the
+// pair flags precomputed: bit0 both hydrophobic, bit1 hydrogen-bond capable.
+__device__ __forceinline__ float eval_ef(float d, int flags, float* dedd) {
+    // gauss1: offset 0, width 0.5
+    const float e1 = __expf(-4.0f * d * d);
+    const float f1 = -8.0f * d * e1;
+    // gauss2: offset 3, width 2
+    const float t2 = (d - 3.0f) * 0.5f;
+    const float e2 = __expf(-t2 * t2);
+    const float f2 = -t2 * e2;
+    float e = -0.035579f * e1 - 0.005156f * e2;
+    float f = -0.035579f * f1 - 0.005156f * f2;
+    // repulsion, d < 0
+    const float dn = fminf(d, 0.0f);
+    e += 0.840245f * dn * dn;
+    f += 0.840245f * 2.0f * dn;
+    // hydrophobic, slope_interp(0.5, 1.5)
+    const float h = fminf(fmaxf(1.5f - d, 0.0f), 1.0f);
+    const bool hin = (flags & 1) && d > 0.5f && d < 1.5f;
+    e += (flags & 1) ? -0.035069f * h : 0.0f;
+    f += hin ? 0.035069f : 0.0f;
+    // hydrogen bond, slope_interp(-0.7, 0)
+    const float b = fminf(fmaxf(-d * (1.0f / 0.7f), 0.0f), 1.0f);
+    const bool bin = (flags & 2) && d > -0.7f && d < 0.0f;
+    e += (flags & 2) ? -0.587439f * b : 0.0f;
+    f += bin ? 0.587439f * (1.0f / 0.7f) : 0.0f;
+    *dedd = f;
+    return e;
+}
+
+__device__ __forceinline__ int pair_flags(int ti, int tj) {
+    int a = c_tflags[ti], b = c_tflags[tj];
+    int hyd = (a & b & 1);
+    int hb = (((a >> 1) & (b >> 2)) | ((a >> 2) & (b >> 1))) & 1;
+    return hyd | (hb << 1);
+}
+
+// normalize_angle (E/src/myutils/mymath.h:38-47).
+__device__ __forceinline__ float normalize_angle(float x) {
+    x = x - 2.0f * kPi * truncf(x * (1.0f / (2.0f * kPi)));  // fmod(x, 2 pi)
+    if (x < -kPi) x += 2.0f * kPi;
+    if (x > kPi) x -= 2.0f * kPi;
+    return x;
+}
+
+struct Mat3 {
+    float m[9];
+};
+
+// Rotation matrix of a unit quaternion (w, x, y, z) = (q.x, q.y, q.z, q.w).
+__device__ __forceinline__ Mat3 quat_to_mat(float4 q) {
+    const float w = q.x, x = q.y, y = q.z, z = q.w;
+    Mat3 R;
+    R.m[0] = w * w + x * x - y * y - z * z; R.m[1] = 2.0f * (x * y - w * z);         R.m[2] = 2.0f * (x * z + w * y);
+    R.m[3] = 2.0f * (x * y + w * z);         R.m[4] = w * w - x * x + y * y - z * z; R.m[5] = 2.0f * (y * z - w * x);
+    R.m[6] = 2.0f * (x * z - w * y);         R.m[7] = 2.0f * (y * z + w * x);         R.m[8] = w * w - x * x - y * y + z * z;
+    return R;
+}
+
+// rotvec_to_quaternion (E/src/geometry/quaternion.h:90-107); returns (w, x, y, z) in (.x, .y, .z, .w).
+__device__ __forceinline__ float4 rotvec_to_quat(float v0, float v1, float v2) {
+    const float th = fsqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    float s, c;
+    __sincosf(th * 0.5f, &s, &c);  // th is a step-sized angle: the hardware sin/cos is accurate here
+    const float k = th > kEps ? s * frcp(th) : 0.0f;
+    return th > kEps ? make_float4(c, v0 * k, v1 * k, v2 * k) : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// acos on [-1, 1] by Abramowitz & Stegun 4.4.46: acos(x) = sqrt(1 - x) P7(x) for
+// x in [0, 1] (|error| <= 2e-8), acos(-x) = pi - acos(x).
+__device__ __forceinline__ float acos_as(float x) {
+    const float ax = fabsf(x);
+    float p = -0.0012624911f;
+    p = p * ax + 0.0066700901f;
+    p = p * ax - 0.0170881256f;
+    p = p * ax + 0.0308918810f;
+    p = p * ax - 0.0501743046f;
+    p = p * ax + 0.0889789874f;
+    p = p * ax - 0.2145988016f;
+    p = p * ax + 1.5707963050f;
+    const float r = fsqrt(1.0f - ax) * p;
+    return x < 0.0f ? 3.14159265358979f - r : r;
+}
+
+// quaternion_to_rotvec (quaternion.h:115-130) of a unit quaternion: theta =
+// 2 acos(w), s = sin(theta / 2) = sqrt(1 - w^2).
+__device__ __forceinline__ float3 quat_to_rotvec(float4 q) {
+    const float w = fminf(fmaxf(q.x, -1.0f), 1.0f);
+    const float th = 2.0f * acos_as(w);
+    const float s = fsqrt(fmaxf(1.0f - w * w, 0.0f));
+    const float k = s > kEps ? th * frcp(s) : 0.0f;
+    return make_float3(q.y * k, q.z * k, q.w * k);
+}
+
+// Orientation gradient of cal_grad_tile (E/src/cuda/bfgs.cuh:122-190): g_k =
+// <dR/dv_k, sum f (x) R^T w> with dR/dv of cal_grad_of_rot_over_vec
+// (E/src/geometry/rotation.h:80-144) evaluated at v', the rotation vector of the
+// same rotation with its angle in [0, pi] (rotvec_to_axis_angle(norm = true),
+// rotation.h:19-46). Since dR/dv'_k R^T = [J(v') e_k]x with J the left Jacobian of
+// SO(3), g = J(v')^T tau, tau = sum_a (x_a - c) x f_a:
+//   g = tau - a v' x tau + b v' x (v' x tau), a = (1 - cos t)/t^2, b = (t - sin t)/t^3.
+__device__ __forceinline__ float3 orient_grad(float v0, float v1, float v2, float t0, float t1, float t2) {
+    const float ang = fsqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    float th = 0.0f, w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+    if (ang > kEps) {
+        th = ang - 2.0f * kPi * floorf(ang * (1.0f / (2.0f * kPi)));  // normalize_angle_2pi
+        float sg = 1.0f;
+        if (th > kPi) { th = 2.0f * kPi - th; sg = -1.0f; }
+        const float k = sg * th * frcp(ang);
+        w0 = v0 * k; w1 = v1 * k; w2 = v2 * k;
+    }
+    const float q = th * th;
+    float a, b;
+    if (th < 0.5f) {  // Taylor series; truncation error below 1e-9 at th = 0.5
+        a = 0.5f - q * (1.0f / 24.0f - q * (1.0f / 720.0f - q * (1.0f / 40320.0f)));
+        b = (1.0f / 6.0f) - q * (1.0f / 120.0f - q * (1.0f / 5040.0f - q * (1.0f / 362880.0f)));
+    } else {
+        float s, c;
+        __sincosf(th, &s, &c);  // th in [0.5, pi]
+        const float iq = frcp(q);
+        a = (1.0f - c) * iq;
+        b = (th - s) * iq * frcp(th);
+    }
+    const float c0 = w1 * t2 - w2 * t1, c1 = w2 * t0 - w0 * t2, c2 = w0 * t1 - w1 * t0;  // v' x tau
+    const float d0 = w1 * c2 - w2 * c1, d1 = w2 * c0 - w0 * c2, d2 = w0 * c1 - w1 * c0;  // v' x (v' x tau)
+    return make_float3(t0 - a * c0 + b * d0, t1 - a * c1 + b * d1, t2 - a * c2 + b * d2);
+}
+
+// Trilinear value and gradient of map `slot` at p; clamped axes have zero derivative.
+__device__ __forceinline__ float grid_eval(const SceneDev& S, int slot, float px, float py, float pz, float& gx,
+                                          float& gy, float& gz) {
+    float g[3] = {(px - S.gorigin[0]) * S.ginv, (py - S.gorigin[1]) * S.ginv, (pz - S.gorigin[2]) * S.ginv};
+    int i0[3];
+    float fr[3];
+    bool cl[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float top = float(S.gn[d] - 1);
+        cl[d] = !(g[d] >= 0.0f && g[d] <= top);
+        float v = fminf(fmaxf(g[d], 0.0f), top);
+        int i = int(floorf(v));
+        i = i > S.gn[d] - 2 ? S.gn[d] - 2 : i;
+        i0[d] = i;
+        fr[d] = v - float(i);
+    }
+    const int nx = S.gn[0], nxy = S.gn[0] * S.gn[1];
+    const float* m = S.maps + (size_t)slot * (size_t)nxy * S.gn[2] + (size_t)i0[2] * nxy + i0[1] * nx + i0[0];
+    float c000 = m[0], c100 = m[1], c010 = m[nx], c110 = m[nx + 1];
+    float c001 = m[nxy], c101 = m[nxy + 1], c011 = m[nxy + nx], c111 = m[nxy + nx + 1];
+    float fx = fr[0], fy = fr[1], fz = fr[2], hx = 1.0f - fx, hy = 1.0f - fy, hz = 1.0f - fz;
+    float v00 = c000 * hx + c100 * fx, v10 = c010 * hx + c110 * fx;
+    float v01 = c001 * hx + c101 * fx, v11 = c011 * hx + c111 * fx;
+    float v0 = v00 * hy + v10 * fy, v1 = v01 * hy + v11 * fy;
+    float dx = ((c100 - c000) * hy + (c110 - c010) * fy) * hz + ((c101 - c001) * hy + (c111 - c011) * fy) * fz;
+    float dy = (v10 - v00) * hz + (v11 - v01) * fz;
+    float dz = v1 - v0;
+    gx = cl[0] ? 0.0f : dx * S.ginv;
+    gy = cl[1] ? 0.0f : dy * S.ginv;
+    gz = cl[2] ? 0.0f : dz * S.ginv;
+    return v0 * hz + v1 * fz;
+}
+
+// ----------------------------------------------------------------- state
+template <int APL>
+struct WPose {
+    float x[APL], y[APL], z[APL];
+    float c0, c1, c2;   // centre (uniform)
+    float r0, r1, r2;   // rot_vec of the last rotation increment (uniform)
+    float dihe;         // lane t: dihedral t
+    float e;            // energy (uniform)
+};
+
+// Per-wave constants of the ligand being processed. The wave's LDS slice
+// (offsets in floats into s_lds) holds: lx float4 coordinates [64*APL], lf
+// pair-force tile [3][64*APL], lo a saved pose, lh the inverse Hessian (row i
+// at lh + i * ld).
+template <int APL>
+struct Ctx {
+    int natom, ntors, npair, nheavy, dim;
+    int tors_off, pair_off;
+    int type[APL];      // XS type; -1 for lanes beyond natom
+    int slot[APL];      // grid map of the atom's type
+    int lx, lf, lo, lh, ld;
+    const LigSetDev* L;
+    const SceneDev* S;
+    int mode;
+    mutable int evals;  // energy evaluations of the current work item
+};
+
+template <int APL>
+__device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const SceneDev& S, int lig, int mode,
+                                         int base, int ld) {
+    const DevLig d = L.ligs[lig];
+    C.natom = d.natom; C.ntors = d.ntors; C.npair = d.npair; C.nheavy = d.nheavy;
+    C.dim = 6 + d.ntors;
+    C.tors_off = d.tors_off; C.pair_off = d.pair_off;
+    C.L = &L; C.S = &S; C.mode = mode;
+    C.evals = 0;
+    C.lx = base;
+    C.lf = base + 4 * kWave * APL;
+    C.lo = base + 7 * kWave * APL;
+    C.lh = base + lds_fixed_floats(APL);
+    C.ld = ld;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        int a = lane() + kWave * k;
+        C.type[k] = a < d.natom ? __float_as_int(L.atom_x0[d.atom_off + a].w) : -1;
+        C.slot[k] = (mode == kGrid && C.type[k] > 0) ? S.slot_tab[C.type[k]] : 0;
+    }
+}
+
+template <int APL>
+__device__ __forceinline__ bool heavy(const Ctx<APL>& C, int k) { return C.type[k] > kTypeH; }
+template <int APL>
+__device__ __forceinline__ bool valid(const Ctx<APL>& C, int k) { return C.type[k] >= 0; }
+
+template <int APL>
+__device__ __forceinline__ float3 atom_pos(const WPose<APL>& P, int a) {
+    const int k = a >> 6, l = a & 63;
+    float3 r = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int kk = 0; kk < APL; kk++)
+        if (kk == k) r = make_float3(rdl(P.x[kk], l), rdl(P.y[kk], l), rdl(P.z[kk], l));
+    return r;
+}
+
+template <int APL>
+__device__ __forceinline__ void centroid(const Ctx<APL>& C, WPose<APL>& P) {
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; k++)
+        if (valid(C, k)) { sx += P.x[k]; sy += P.y[k]; sz += P.z[k]; }
+    const float inv = 1.0f / float(C.natom);
+    P.c0 = uni(wave_sum(sx) * inv);
+    P.c1 = uni(wave_sum(sy) * inv);
+    P.c2 = uni(wave_sum(sz) * inv);
+}
+
+// Pose slots in the wave's LDS slice: slot 0 parks the BFGS start pose, slot 1
+// the accepted MC pose. A slot holds x, y, z per atom and the dihedrals per lane,
+// then c, rot_vec and the energy.
+template <int APL>
+__device__ __forceinline__ int pose_slot(const Ctx<APL>& C, int slot) {
+    return C.lo + slot * (kWave * (3 * APL + 1) + 8);
+}
+
+template <int APL>
+__device__ __forceinline__ void save_pose(const Ctx<APL>& C, const WPose<APL>& P, int slot = 0) {
+    const int ln = lane();
+    const int lo = pose_slot(C, slot);
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        s_lds[lo + (3 * k) * kWave + ln] = P.x[k];
+        s_lds[lo + (3 * k + 1) * kWave + ln] = P.y[k];
+        s_lds[lo + (3 * k + 2) * kWave + ln] = P.z[k];
+    }
+    s_lds[lo + 3 * APL * kWave + ln] = P.dihe;
+    const int u = lo + (3 * APL + 1) * kWave;
+    if (ln == 0) {
+        s_lds[u] = P.c0; s_lds[u + 1] = P.c1; s_lds[u + 2] = P.c2;
+        s_lds[u + 3] = P.r0; s_lds[u + 4] = P.r1; s_lds[u + 5] = P.r2;
+        s_lds[u + 6] = P.e;
+    }
+}
+
+template <int APL>
+__device__ __forceinline__ void restore_pose(const Ctx<APL>& C, WPose<APL>& P, int slot = 0) {
+    wave_lds_fence();
+    const int ln = lane();
+    const int lo = pose_slot(C, slot);
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        P.x[k] = s_lds[lo + (3 * k) * kWave + ln];
+        P.y[k] = s_lds[lo + (3 * k + 1) * kWave + ln];
+        P.z[k] = s_lds[lo + (3 * k + 2) * kWave + ln];
+    }
+    P.dihe = s_lds[lo + 3 * APL * kWave + ln];
+    const int u = lo + (3 * APL + 1) * kWave;
+    P.c0 = uni(s_lds[u]); P.c1 = uni(s_lds[u + 1]); P.c2 = uni(s_lds[u + 2]);
+    P.r0 = uni(s_lds[u + 3]); P.r1 = uni(s_lds[u + 4]); P.r2 = uni(s_lds[u + 5]);
+    P.e = uni(s_lds[u + 6]);
+}
+
+// clamp_by_ranges (E/src/myutils/mymath.h:133-158) over the torsion's ranges.
+__device__ __forceinline__ float clamp_ranges(const LigSetDev& L, int tg, float ang) {
+    const int2 rr = L.tors_range[tg];
+    for (int i = 0; i < rr.y; i++) {
+        const float2 r = L.ranges[rr.x + i];
+        if (ang >= r.x && ang <= r.y) return ang;
+    }
+    float best = ang, md = 100.0f;
+    for (int i = 0; i < rr.y; i++) {
+        const float2 r = L.ranges[rr.x + i];
+        const float dl = fabsf(ang - r.x), dh = fabsf(ang - r.y);
+        if (dl < md) { md = dl; best = r.x; }
+        if (dh < md) { md = dh; best = r.y; }
+    }
+    return best;
+}
+
+// One torsion step, apply_grad_update_dihe_tile (E/src/cuda/bfgs.cuh:443-527);
+// the centroid is refreshed by the caller after the last torsion (only the
+// final centroid is ever read, bfgs.cuh:531-548, 576-578).
+template <int APL>
+__device__ __forceinline__ void apply_torsion(const Ctx<APL>& C, WPose<APL>& P, int t, float inc_raw) {
+    const int tg = C.tors_off + t;
+    const float dt = rdl(P.dihe, t);
+    float inc = normalize_angle(inc_raw);
+    const float nw = uni(clamp_ranges(*C.L, tg, normalize_angle(dt + inc)));
+    inc = nw - dt;
+    if (lane() == t) P.dihe = nw;
+    const int2 ax = C.L->tors_axis[tg];
+    const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
+    float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
+    const float l = fsqrt(ux * ux + uy * uy + uz * uz);
+    if (l <= kEps) { ux = 1.0f; uy = 0.0f; uz = 0.0f; }
+    else { const float il = frcp(l); ux *= il; uy *= il; uz *= il; }
+    float s, c;
+    __sincosf(inc, &s, &c);
+    ux = uni(ux); uy = uni(uy); uz = uni(uz); s = uni(s); c = uni(c);
+    const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const unsigned long long mk = mask[k];
+        if ((mk >> lane()) & 1ull) {
+            const float vx = P.x[k] - xt.x, vy = P.y[k] - xt.y, vz = P.z[k] - xt.z;
+            const float m = (1.0f - c) * (ux * vx + uy * vy + uz * vz);
+            const float cxv = uy * vz - uz * vy, cyv = uz * vx - ux * vz, czv = ux * vy - uy * vx;
+            P.x[k] = c * vx + m * ux + s * cxv + xt.x;
+            P.y[k] = c * vy + m * uy + s * cyv + xt.y;
+            P.z[k] = c * vz + m * uz + s * czv + xt.z;
+        }
+    }
+}
+
+// Rigid rotation about o by q, then translation so that o maps to n.
+template <int APL>
+__device__ __forceinline__ void rigid_move(const Ctx<APL>& C, WPose<APL>& P, float4 q, float ox, float oy, float oz,
+                                           float nx, float ny, float nz) {
+    Mat3 R = quat_to_mat(q);
+#pragma unroll
+    for (int i = 0; i < 9; i++) R.m[i] = uni(R.m[i]);
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        if (!valid(C, k)) continue;
+        const float wx = P.x[k] - ox, wy = P.y[k] - oy, wz = P.z[k] - oz;
+        P.x[k] = R.m[0] * wx + R.m[1] * wy + R.m[2] * wz + nx;
+        P.y[k] = R.m[3] * wx + R.m[4] * wy + R.m[5] * wz + ny;
+        P.z[k] = R.m[6] * wx + R.m[7] * wy + R.m[8] * wz + nz;
+    }
+}
+
+// apply_grad_update_pose_tile (bfgs.cuh:560-618): torsions in index order, then
+// centre += alpha * dir[0..2] and a rotation by q(alpha * dir[3..5]) about the
+// old centre; rot_vec := alpha * dir[3..5]. dir is lane-distributed.
+template <int APL>
+__device__ __forceinline__ void apply_update(const Ctx<APL>& C, WPose<APL>& P, float dir, float alpha) {
+#pragma unroll 1
+    for (int t = 0; t < C.ntors; t++) apply_torsion(C, P, t, rdl(dir, 6 + t) * alpha);
+    if (C.ntors > 0) centroid(C, P);
+    const float o0 = P.c0, o1 = P.c1, o2 = P.c2;
+    P.c0 = uni(o0 + alpha * rdl(dir, 0));
+    P.c1 = uni(o1 + alpha * rdl(dir, 1));
+    P.c2 = uni(o2 + alpha * rdl(dir, 2));
+    const float v0 = uni(rdl(dir, 3) * alpha), v1 = uni(rdl(dir, 4) * alpha), v2 = uni(rdl(dir, 5) * alpha);
+    const float4 q = rotvec_to_quat(v0, v1, v2);
+    P.r0 = v0; P.r1 = v1; P.r2 = v2;
+    rigid_move(C, P, q, o0, o1, o2, P.c0, P.c1, P.c2);
+}
+
+// Per-atom energy gradient (-force) of one evaluation, atom a on lane a % 64.
+template <int APL>
+struct Forces {
+    float fx[APL], fy[APL], fz[APL];
+};
+
+// cal_e_f_tile (bfgs.cuh:221-394): energy under the curl cap v_cap and the
+// per-atom gradient F.
+template <int APL>
+__device__ float energy_forces(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope, Forces<APL>& F) {
+    const SceneDev& S = *C.S;
+    const int ln = lane();
+    const int n3 = kWave * APL;
+    float* fx = F.fx;
+    float* fy = F.fy;
+    float* fz = F.fz;
+    float e = 0.0f;
+    C.evals++;
+    // coordinates to LDS for the pair loop; zero the pair-force tile
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const int a = ln + kWave * k;
+        reinterpret_cast<float4*>(s_lds + C.lx)[a] = make_float4(P.x[k], P.y[k], P.z[k], 0.0f);
+        s_lds[C.lf + a] = 0.0f;
+        s_lds[C.lf + n3 + a] = 0.0f;
+        s_lds[C.lf + 2 * n3 + a] = 0.0f;
+    }
+    // inter-molecular term with the per-atom curl, and the box penalty
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        fx[k] = fy[k] = fz[k] = 0.0f;
+        if (!heavy(C, k)) continue;
+        const int ti = C.type[k];
+        float ea = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if (C.mode == kGrid) {
+            ea = grid_eval(S, C.slot[k], P.x[k], P.y[k], P.z[k], gx, gy, gz);
+        } else {
+            const float ri = c_vdw[ti];
+#pragma unroll 1
+            for (int j = 0; j < S.nrec; j++) {
+                const float4 r = S.rec[j];
+                const float rx = r.x - P.x[k], ry = r.y - P.y[k], rz = r.z - P.z[k];
+                const float rr = rx * rx + ry * ry + rz * rz;
+                if (rr < kR2Cut) {
+                    const int tj = __float_as_int(r.w);
+                    float d = fsqrt(rr), dd;
+                    ea += eval_ef(d - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
+                    d = fmaxf(d, kEps);
+                    const float fdr = dd * frcp(d);
+                    gx -= fdr * rx; gy -= fdr * ry; gz -= fdr * rz;
+                }
+            }
+        }
+        if (ea > 0.0f) {
+            float s = v_cap * frcp(v_cap + ea);
+            ea *= s;
+            s *= s;
+            gx *= s; gy *= s; gz *= s;
+        }
+        e += ea;
+        // cal_box_penalty (bfgs.cuh:196-218)
+        float pen = 0.0f;
+        if (P.x[k] < S.box_lo[0]) { pen += S.box_lo[0] - P.x[k]; gx -= slope; }
+        if (P.x[k] > S.box_hi[0]) { pen += P.x[k] - S.box_hi[0]; gx += slope; }
+        if (P.y[k] < S.box_lo[1]) { pen += S.box_lo[1] - P.y[k]; gy -= slope; }
+        if (P.y[k] > S.box_hi[1]) { pen += P.y[k] - S.box_hi[1]; gy += slope; }
+        if (P.z[k] < S.box_lo[2]) { pen += S.box_lo[2] - P.z[k]; gz -= slope; }
+        if (P.z[k] > S.box_hi[2]) { pen += P.z[k] - S.box_hi[2]; gz += slope; }
+        e += pen * slope;
+        fx[k] = gx; fy[k] = gy; fz[k] = gz;
+    }
+    wave_lds_fence();
+    // intra-molecular pairs with the per-pair curl; each pair once
+    const uint2* pr = C.L->pairs + C.pair_off;
+    const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
+#pragma unroll 1
+    for (int p = ln; p < C.npair; p += kWave) {
+        const uint2 rec = pr[p];
+        const int i = rec.x & 255, j = (rec.x >> 8) & 255, fl = (rec.x >> 16) & 3;
+        const float rs = __uint_as_float(rec.y);
+        const float4 xi = lx4[i], xj = lx4[j];
+        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+        const float rr = rx * rx + ry * ry + rz * rz;
+        if (rr < kR2Cut) {
+            float d = fsqrt(rr), dd;
+            float ep = eval_ef(d - rs, fl, &dd);
+            d = fmaxf(d, kEps);
+            const float fdr = dd * frcp(d);
+            float px = -fdr * rx, py = -fdr * ry, pz = -fdr * rz;
+            // per-pair curl, branch-free
+            float s = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
+            ep *= s;
+            s *= s;
+            px *= s; py *= s; pz *= s;
+            e += ep;
+            atomicAdd(&s_lds[C.lf + i], px);
+            atomicAdd(&s_lds[C.lf + n3 + i], py);
+            atomicAdd(&s_lds[C.lf + 2 * n3 + i], pz);
+            atomicAdd(&s_lds[C.lf + j], -px);
+            atomicAdd(&s_lds[C.lf + n3 + j], -py);
+            atomicAdd(&s_lds[C.lf + 2 * n3 + j], -pz);
+        }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const int a = ln + kWave * k;
+        fx[k] += s_lds[C.lf + a];
+        fy[k] += s_lds[C.lf + n3 + a];
+        fz[k] += s_lds[C.lf + 2 * n3 + a];
+    }
+    return uni(wave_sum(e));
+}
+
+// cal_grad_tile (bfgs.cuh:69-193): the pose gradient from the per-atom gradient,
+// returned lane-distributed (lane k = component k).
+template <int APL>
+__device__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& P, const Forces<APL>& F) {
+    const int ln = lane();
+    const float* fx = F.fx;
+    const float* fy = F.fy;
+    const float* fz = F.fz;
+
+    // gradient (cal_grad_tile, bfgs.cuh:69-193): centre = sum of heavy-atom gradients;
+    // orientation from the torque about the centre (see orient_grad); torsions below.
+    float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, tqx = 0.0f, tqy = 0.0f, tqz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        if (!heavy(C, k)) continue;
+        sgx += fx[k]; sgy += fy[k]; sgz += fz[k];
+        const float wx = P.x[k] - P.c0, wy = P.y[k] - P.c1, wz = P.z[k] - P.c2;
+        tqx += wy * fz[k] - wz * fy[k];
+        tqy += wz * fx[k] - wx * fz[k];
+        tqz += wx * fy[k] - wy * fx[k];
+    }
+    float gout = 0.0f, v;
+    v = wave_sum(sgx); gout = ln == 0 ? v : gout;
+    v = wave_sum(sgy); gout = ln == 1 ? v : gout;
+    v = wave_sum(sgz); gout = ln == 2 ? v : gout;
+    {
+        const float3 og = orient_grad(P.r0, P.r1, P.r2, wave_sum(tqx), wave_sum(tqy), wave_sum(tqz));
+        gout = ln == 3 ? og.x : gout;
+        gout = ln == 4 ? og.y : gout;
+        gout = ln == 5 ? og.z : gout;
+    }
+#pragma unroll 1
+    for (int t = 0; t < C.ntors; t++) {
+        const int tg = C.tors_off + t;
+        const int2 ax = C.L->tors_axis[tg];
+        const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
+        float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
+        const float inv = frsq(ux * ux + uy * uy + uz * uz);
+        ux = uni(ux * inv); uy = uni(uy * inv); uz = uni(uz * inv);
+        const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
+        float st = 0.0f;
+#pragma unroll
+        for (int k = 0; k < APL; k++) {
+            if (((mask[k] >> ln) & 1ull) && heavy(C, k)) {
+                const float lx = P.x[k] - xt.x, ly = P.y[k] - xt.y, lz = P.z[k] - xt.z;
+                const float cx = ly * fz[k] - lz * fy[k], cy = lz * fx[k] - lx * fz[k], cz = lx * fy[k] - ly * fx[k];
+                st += cx * ux + cy * uy + cz * uz;
+            }
+        }
+        v = wave_sum(st);
+        gout = ln == 6 + t ? v : gout;
+    }
+    return gout;
+}
+
+// cal_e_grad_tile (bfgs.cuh:397-410).
+template <int APL>
+__device__ __forceinline__ float energy_grad(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope,
+                                             float& g) {
+    Forces<APL> F;
+    const float e = energy_forces(C, P, v_cap, slope, F);
+    g = pose_grad(C, P, F);
+    return e;
+}
+
+// line_search_tile (bfgs.cuh:623-670).
+template <int APL>
+__device__ void line_search(const Ctx<APL>& C, const WPose<APL>& x, float g, float p, float e0, float v_cap,
+                            float slope, WPose<APL>& x_new, float& g_new, float& e_out, float& alpha_out) {
+    const float pg = uni(wave_sum(p * g));
+    float alpha = 1.0f, e_new = 0.0f;
+    Forces<APL> F;
+    for (int trial = 0; trial < 10; trial++) {
+        x_new = x;
+        apply_update(C, x_new, p, alpha);
+        e_new = energy_forces(C, x_new, v_cap, slope, F);
+        if (e_new - e0 < 1e-4f * alpha * pg) break;
+        alpha *= 0.5f;
+    }
+    // only the gradient of the trial the search ends on is ever used (bfgs.cuh:641-667)
+    g_new = pose_grad(C, x_new, F);
+    e_out = e_new;
+    alpha_out = alpha;
+}
+
+// bfgs_tile (bfgs.cuh:729-829): the inverse Hessian lives in the wave's LDS
+// tile (lane i owns row i) and the start pose is parked in LDS for the
+// restore-on-failure at the end.
+template <int APL>
+__device__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_cap, float slope) {
+    const int ln = lane(), dim = C.dim;
+    const int hrow = C.lh + ln * C.ld;
+    const bool row = ln < dim;
+    float g;
+    const float E_ori = energy_grad(C, x, v_cap, slope, g);
+    float E = E_ori, E1 = 0.0f, alpha = 0.0f;
+    if (row)
+        for (int j = 0; j < dim; j++) s_lds[hrow + j] = (j == ln) ? 1.0f : 0.0f;
+    float g_new = g;
+    WPose<APL> x_new = x;
+    save_pose(C, x);
+    float p = g;
+    for (int step = 0; step < max_steps; step++) {
+        float s = 0.0f;
+        if (row)
+            for (int j = 0; j < dim; j++) s += s_lds[hrow + j] * rdl(g, j);
+        p = row ? -s : 0.0f;
+        line_search(C, x, g, p, E, v_cap, slope, x_new, g_new, E1, alpha);
+        const float y = g_new - g;
+        E = E1;
+        x = x_new;
+        const float gg = wave_sum(g * g);
+        if (sqrtf(gg) < 1e-5f) break;
+        g = g_new;
+        if (step == 0) {
+            const float yy = uni(wave_sum(y * y));
+            if (fabsf(yy) > kEps) {
+                const float yp0 = wave_sum(y * p);
+                if (row) s_lds[hrow + ln] = alpha * yp0 / yy;
+            }
+        }
+        const float yp = uni(wave_sum(y * p));
+        if (alpha * yp < kEps) continue;
+        float hy = 0.0f;
+        if (row)
+            for (int j = 0; j < dim; j++) hy += s_lds[hrow + j] * rdl(y, j);
+        const float mhy = row ? -hy : 0.0f;
+        const float yhy = -wave_sum(y * mhy);
+        const float r = 1.0f / (alpha * yp);
+        const float a1 = uni(alpha * r), a2 = uni(alpha * alpha * (r * r * yhy + r));
+        if (row)
+            for (int j = 0; j < dim; j++) {
+                const float pj = rdl(p, j), mj = rdl(mhy, j);
+                s_lds[hrow + j] += a1 * (mhy * pj + mj * p) + a2 * p * pj;
+            }
+    }
+    if (E > E_ori) {
+        restore_pose(C, x);
+        E = E_ori;
+    }
+    x.e = uni(E);
+}
+
+// ------------------------------------------------------------------ RNG
+// Philox4x32-10 (Salmon et al., SC'11); the stream of a chain is keyed by
+// (seed, ligand index) with counter (chain, draw, 0x55443244, 0).
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+__device__ __forceinline__ float u01(uint32_t u) { return float((u >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+struct Rng {
+    uint32_t k0, k1, chain, draw;
+    __device__ uint4 next() { return philox(chain, draw++, 0x55443244u, 0u, k0, k1); }
+    __device__ uint4 at(uint32_t d) const { return philox(chain, d, 0x55443244u, 0u, k0, k1); }
+};
+
+// gen_4_rand_in_sphere (E/src/cuda/common.cuh:90-103); the acceptance test is
+// evaluated with rounded (uncontracted) float operations.
+__device__ __forceinline__ float3 rand_in_ball(Rng& rng) {
+    for (;;) {
+        const uint4 b = rng.next();
+        const float a0 = (u01(b.x) - 0.5f) * 2.0f, a1 = (u01(b.y) - 0.5f) * 2.0f, a2 = (u01(b.z) - 0.5f) * 2.0f;
+        const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(a0, a0), __fmul_rn(a1, a1)), __fmul_rn(a2, a2));
+        if (r2 < 1.0f) return make_float3(a0, a1, a2);
+    }
+}
+
+// gyration_radius (common.cuh:106-122).
+template <int APL>
+__device__ __forceinline__ float gyration(const Ctx<APL>& C, const WPose<APL>& P) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; k++)
+        if (heavy(C, k)) {
+            const float dx = P.x[k] - P.c0, dy = P.y[k] - P.c1, dz = P.z[k] - P.c2;
+            s += dx * dx + dy * dy + dz * dz;
+        }
+    s = wave_sum(s);
+    const float r = C.nheavy > 0 ? sqrtf(s / float(C.nheavy)) : 0.0f;
+    return uni(r > 0.0f ? r : 1.0f);
+}
+
+// What a mutation did: kind 0 translation, 1 rotation, 2 torsion; magnitude
+// |d| / 2 A, rotation angle / pi or |torsion change| / pi; moved = fraction of
+// heavy atoms displaced. The plain MC ignores it; the learned proposal feeds it
+// to its network.
+struct Move {
+    int kind;
+    float mag, moved;
+};
+
+// mutate_pose_tile (E/src/cuda/mc.cu:146-256).
+template <int APL>
+__device__ Move mutate(const Ctx<APL>& C, WPose<APL>& P, Rng& rng) {
+    const SceneDev& S = *C.S;
+    const uint4 b = rng.next();
+    const int which = int(b.x % uint32_t(2 + C.ntors));
+    Move mv;
+    if (which == 0) {
+        const float3 a = rand_in_ball(rng);
+        const float d0 = uni(fminf(fmaxf(2.0f * a.x + P.c0, S.box_lo[0]), S.box_hi[0]) - P.c0);
+        const float d1 = uni(fminf(fmaxf(2.0f * a.y + P.c1, S.box_lo[1]), S.box_hi[1]) - P.c1);
+        const float d2 = uni(fminf(fmaxf(2.0f * a.z + P.c2, S.box_lo[2]), S.box_hi[2]) - P.c2);
+        P.c0 = uni(P.c0 + d0); P.c1 = uni(P.c1 + d1); P.c2 = uni(P.c2 + d2);
+#pragma unroll
+        for (int k = 0; k < APL; k++)
+            if (valid(C, k)) { P.x[k] += d0; P.y[k] += d1; P.z[k] += d2; }
+        mv.kind = 0;
+        mv.mag = 0.5f * sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+        mv.moved = 1.0f;
+    } else if (which == 1) {
+        const float3 a = rand_in_ball(rng);
+        const float gr = gyration(C, P);
+        const float v0 = uni(2.0f / gr * a.x), v1 = uni(2.0f / gr * a.y), v2 = uni(2.0f / gr * a.z);
+        const float4 q = rotvec_to_quat(v0, v1, v2);
+        P.r0 = v0; P.r1 = v1; P.r2 = v2;
+        rigid_move(C, P, q, P.c0, P.c1, P.c2, P.c0, P.c1, P.c2);
+        mv.kind = 1;
+        mv.mag = sqrtf(v0 * v0 + v1 * v1 + v2 * v2) * (1.0f / kPi);
+        mv.moved = 1.0f;
+    } else {
+        const int t = which - 2;
+        const uint4 bb = rng.next();
+        const int2 rr = C.L->tors_range[C.tors_off + t];
+        int i = int(ceilf(__fsub_rn(__fmul_rn(u01(bb.x), float(rr.y)), 1.0f)));
+        i = i < 0 ? 0 : (i > rr.y - 1 ? rr.y - 1 : i);
+        const float2 r = C.L->ranges[rr.x + i];
+        const float target = r.x + (r.y - r.x) * u01(bb.y);
+        const float inc = target - rdl(P.dihe, t);
+        apply_torsion(C, P, t, inc);
+        centroid(C, P);
+        const unsigned long long* mask = C.L->tors_mask + (size_t)(C.tors_off + t) * kMaxApl;
+        int nmoved = 0;
+#pragma unroll
+        for (int k = 0; k < APL; k++) nmoved += __popcll(mask[k] & __ballot(heavy(C, k)));
+        mv.kind = 2;
+        mv.mag = fabsf(normalize_angle(inc)) * (1.0f / kPi);
+        mv.moved = C.nheavy > 0 ? float(nmoved) / float(C.nheavy) : 0.0f;
+    }
+    return mv;
+}
+
+// randomize_pose_tile (mc.cu:18-104), non-constraint branch: centre drawn in
+// the box, super-Fibonacci orientation by chain index, each torsion drawn in a
+// random allowed range; applied through apply_update with alpha = 1.
+template <int APL>
+__device__ void randomize(const Ctx<APL>& C, WPose<APL>& P, Rng& rng, int idx, int n) {
+    const SceneDev& S = *C.S;
+    const int ln = lane();
+    const uint4 b = rng.next();
+    const float gc0 = (S.box_lo[0] + (S.box_hi[0] - S.box_lo[0]) * u01(b.x)) - P.c0;
+    const float gc1 = (S.box_lo[1] + (S.box_hi[1] - S.box_lo[1]) * u01(b.y)) - P.c1;
+    const float gc2 = (S.box_lo[2] + (S.box_hi[2] - S.box_lo[2]) * u01(b.z)) - P.c2;
+    // super-Fibonacci sample idx of n (Alexa, CVPR'22): angles 2 pi s / phi and
+    // 2 pi s / psi, reduced to one turn before the hardware sin / cos
+    const float s = float(idx) + 0.5f;
+    const float r = fsqrt(s / float(n)), R = fsqrt(1.0f - s / float(n));
+    const float fa = s * (1.0f / 1.4142135623730951f), fb = s * (1.0f / 1.5337511687552043f);
+    float sa, ca, sb, cb;
+    __sincosf(2.0f * kPi * (fa - floorf(fa)), &sa, &ca);
+    __sincosf(2.0f * kPi * (fb - floorf(fb)), &sb, &cb);
+    const float3 rv = quat_to_rotvec(make_float4(r * sa, r * ca, R * sb, R * cb));
+    // torsion t draws block (draw + t) on lane t: a range, then a value in it
+    float tv = 0.0f;
+    if (ln < C.ntors) {
+        const uint4 bt = rng.at(rng.draw + ln);
+        const int2 rr = C.L->tors_range[C.tors_off + ln];
+        int j = int(__fmul_rn(u01(bt.x), float(rr.y)));
+        j = j > rr.y - 1 ? rr.y - 1 : j;
+        const float2 rg = C.L->ranges[rr.x + j];
+        tv = (rg.x + (rg.y - rg.x) * u01(bt.y)) - P.dihe;
+    }
+    rng.draw += C.ntors;
+    // direction vector: lanes 0..5 rigid part, lane 6 + t torsion t
+    const float sh = __shfl(tv, (ln - 6) & 63);
+    float d = (ln >= 6 && ln < 6 + C.ntors) ? sh : 0.0f;
+    d = ln == 0 ? gc0 : d;
+    d = ln == 1 ? gc1 : d;
+    d = ln == 2 ? gc2 : d;
+    d = ln == 3 ? rv.x : d;
+    d = ln == 4 ? rv.y : d;
+    d = ln == 5 ? rv.z : d;
+    apply_update(C, P, d, 1.0f);
+}
+
+// ---------------------------------------------------------- pose buffers
+template <int APL>
+__device__ __forceinline__ void load_pose(const LigSetDev& L, const PoseBufs& B, int lig, int chain, WPose<APL>& P) {
+    const DevLig d = L.ligs[lig];
+    const size_t xo = ((size_t)d.atom_off * B.nchain + (size_t)chain * d.natom) * 3;
+    const size_t po = (size_t)lig * B.nchain + chain;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const int a = lane() + kWave * k;
+        if (a < d.natom) { P.x[k] = B.x[xo + 3 * a]; P.y[k] = B.x[xo + 3 * a + 1]; P.z[k] = B.x[xo + 3 * a + 2]; }
+        else { P.x[k] = P.y[k] = P.z[k] = 0.0f; }
+    }
+    P.c0 = B.c[po * 3]; P.c1 = B.c[po * 3 + 1]; P.c2 = B.c[po * 3 + 2];
+    P.r0 = B.rv[po * 3]; P.r1 = B.rv[po * 3 + 1]; P.r2 = B.rv[po * 3 + 2];
+    const size_t dof = (size_t)d.tors_off * B.nchain + (size_t)chain * d.ntors;
+    P.dihe = lane() < d.ntors ? B.dihe[dof + lane()] : 0.0f;
+    P.e = B.e[po];
+}
+
+template <int APL>
+__device__ __forceinline__ void store_pose(const LigSetDev& L, const PoseBufs& B, int lig, int chain,
+                                           const WPose<APL>& P) {
+    const DevLig d = L.ligs[lig];
+    const size_t xo = ((size_t)d.atom_off * B.nchain + (size_t)chain * d.natom) * 3;
+    const size_t po = (size_t)lig * B.nchain + chain;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const int a = lane() + kWave * k;
+        if (a < d.natom) { B.x[xo + 3 * a] = P.x[k]; B.x[xo + 3 * a + 1] = P.y[k]; B.x[xo + 3 * a + 2] = P.z[k]; }
+    }
+    const int ln = lane();
+    if (ln == 0) { B.c[po * 3] = P.c0; B.rv[po * 3] = P.r0; B.e[po] = P.e; }
+    if (ln == 1) { B.c[po * 3 + 1] = P.c1; B.rv[po * 3 + 1] = P.r1; }
+    if (ln == 2) { B.c[po * 3 + 2] = P.c2; B.rv[po * 3 + 2] = P.r2; }
+    const size_t dof = (size_t)d.tors_off * B.nchain + (size_t)chain * d.ntors;
+    if (ln < d.ntors) B.dihe[dof + ln] = P.dihe;
+}
+
+// Input conformation of a ligand: coordinates, dihedrals, centroid, rot_vec = 0,
+// energy 999 (E/src/cuda/task.cu:94-108).
+template <int APL>
+__device__ __forceinline__ void initial_pose(const Ctx<APL>& C, const LigSetDev& L, int lig, WPose<APL>& P) {
+    const DevLig d = L.ligs[lig];
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const int a = lane() + kWave * k;
+        if (a < d.natom) { const float4 v = L.atom_x0[d.atom_off + a]; P.x[k] = v.x; P.y[k] = v.y; P.z[k] = v.z; }
+        else { P.x[k] = P.y[k] = P.z[k] = 0.0f; }
+    }
+    P.dihe = lane() < d.ntors ? L.tors_dihe0[d.tors_off + lane()] : 0.0f;
+    centroid(C, P);
+    P.r0 = P.r1 = P.r2 = 0.0f;
+    P.e = 999.0f;
+}
+
+}  // namespace ud2

@@ -138,6 +138,81 @@ struct GridArgs {
     float* maps;
 };
 
+// ---------------------------------------------------------- learned proposal
+constexpr int kFeat = 80;     // network inputs of one candidate
+constexpr int kPooled = 64;   // 8 feature classes x 8 statistics
+constexpr int kHidden = 256;
+constexpr int kGroup = 16;    // candidates of one MC state = MFMA tile width
+constexpr int kMaxCand = 16;
+constexpr int kMlpNParam = kFeat * kHidden + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
+constexpr int kMlpNNorm = 2 * kFeat;
+// canonical parameter offsets: W1[80][256], b1, W2[256][256], b2, w3, b3; then mu[80], inv_sigma[80]
+constexpr int kOffW1 = 0, kOffB1 = kFeat * kHidden, kOffW2 = kOffB1 + kHidden, kOffB2 = kOffW2 + kHidden * kHidden,
+              kOffW3 = kOffB2 + kHidden, kOffB3 = kOffW3 + kHidden;
+
+// Device copies of the network used by the MFMA kernels.
+struct MlpDev {
+    const uint16_t* w1t;  // [256][80] bf16, W1 transposed (hidden-major)
+    const uint16_t* w2t;  // [256 out][256 in] bf16
+    const uint16_t* w2;   // [256 in][256 out] bf16 (backward)
+    const float* b1;      // [256]
+    const float* b2;      // [256]
+    const float* w3;      // [256], values rounded to bf16
+    const float* b3;      // [1]
+    const float* mu;      // [80]
+    const float* inv_sigma;  // [80]
+};
+
+// Training buffers of one minibatch of n_groups groups (rows = 16 * n_groups).
+struct MlpTrain {
+    const uint16_t* x;       // dataset features [*][80]
+    const int32_t* groups;   // dataset group of each minibatch group (null: identity)
+    int32_t n_groups;
+    const float* dy;         // [rows] d loss / d y_hat
+    uint16_t* xn_fm;         // [80][rows] normalised inputs, feature-major
+    uint16_t* a1_fm;         // [256][rows]
+    uint16_t* dz1_fm;        // [256][rows]
+    uint16_t* dz2_fm;        // [256][rows]
+    float* part;             // [n_groups][kPartStride]: dw3, db2, db1, db3 partial sums
+};
+constexpr int kPartStride = 3 * kHidden + 4;
+
+enum LmcPolicy : int32_t { kPolicyPlain = 0, kPolicyExact = 1, kPolicyLearned = 2, kPolicyCollect = 3 };
+
+// One round of the learned-proposal MC over every chain of a launch class.
+struct LmcArgs {
+    LigSetDev L;
+    SceneDev S;
+    PoseBufs best;           // best pose of each chain (the result)
+    PoseBufs acc;            // accepted pose of each chain
+    const int32_t* order;
+    int32_t n_order;
+    int32_t* counter;        // work-queue head of this launch, zero at launch
+    int32_t mode;            // inter term of the search energy
+    int32_t opt_steps, randomize;
+    uint32_t seed;
+    int32_t lig_index_base;
+    int32_t lds_floats_per_wave;
+    int32_t hdim_ld;
+    int32_t policy, n_cand, top_k, step;
+    uint32_t* draw;          // [n_pose] next Philox block of each chain
+    uint32_t* cand_draw;     // [n_pose][16] first block of each candidate's mutation
+    uint16_t* x;             // [n_pose][16][80] candidate features
+    const float* pred;       // [n_pose][16] network scores
+    float* label;            // collect: [n_pose][16] energies after BFGS
+    unsigned long long* stats;
+};
+
+struct FeatArgs {
+    LigSetDev L;
+    SceneDev S;
+    PoseBufs P;
+    const int32_t* sel;
+    int32_t n_sel;
+    uint16_t* x;             // [n_sel][80]
+    int32_t lds_floats_per_wave;
+};
+
 // Launchers (ud2_kernels.hip); each returns a hipError_t.
 hipError_t launch_grid_build(const GridArgs& a, hipStream_t s);
 hipError_t launch_mc(McArgs a, int apl, int n_blocks, hipStream_t s);
@@ -151,5 +226,26 @@ int mc_blocks_per_cu(int apl, int lds_bytes_per_block);
 // coordinates + pair forces (7 * 64 * apl) and two parked poses (64 * (3 * apl + 1) + 8 each)
 __host__ __device__ constexpr int lds_fixed_floats(int apl) { return 7 * kWave * apl + 2 * (kWave * (3 * apl + 1) + 8); }
 int posevec_blocks_per_cu(int apl, int lds_bytes_per_block);
+// LDS floats per wave of the proposal kernel: the fixed slice and 8 floats per atom of feature scratch.
+__host__ __device__ constexpr int lds_prop_floats(int apl) { return lds_fixed_floats(apl) + 8 * kWave * apl; }
+
+// Learned proposal (ud2_learned.hip) and network (ud2_mlp.hip) launchers.
+hipError_t launch_lmc_init(LmcArgs a, int apl, int n_blocks, hipStream_t s);
+hipError_t launch_lmc_propose(LmcArgs a, int apl, int n_blocks, hipStream_t s);
+hipError_t launch_lmc_step(LmcArgs a, int apl, int n_blocks, hipStream_t s);
+hipError_t launch_features(FeatArgs a, int apl, hipStream_t s);
+int lmc_blocks_per_cu(int kernel, int apl, int lds_bytes_per_block);
+hipError_t launch_mlp_forward(const MlpDev& W, const uint16_t* x, const int32_t* groups, int64_t n_groups, float* y,
+                              hipStream_t s);
+hipError_t launch_mlp_backward(const MlpDev& W, const MlpTrain& T, hipStream_t s);
+hipError_t launch_gemm_nt(const uint16_t* A, const uint16_t* B, int M, int N, int K, int splits, float* part,
+                          float* C, hipStream_t s);
+hipError_t launch_mlp_reduce_parts(const float* part, int n_groups, float* grad, hipStream_t s);
+hipError_t launch_group_loss(const float* yhat, const float* label, const int32_t* groups, int n_groups, float* dy,
+                             float* loss_parts, float* loss, hipStream_t s);
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int n, float lr, float beta1, float beta2,
+                       float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_mlp_pack(const float* params, uint16_t* w1t, uint16_t* w2t, uint16_t* w2, float* w3r,
+                           hipStream_t s);
 
 }  // namespace ud2

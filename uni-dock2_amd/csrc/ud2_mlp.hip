@@ -1,0 +1,358 @@
+// BF16 MFMA kernels of the learned MC proposal network (no reference
+// counterpart): inference, backward pass, weight-gradient GEMM, loss, Adam.
+//
+// Network: y = w3 . relu(W2^T a1 + b2) + b3, a1 = bf16(relu(W1^T xn + b1)),
+// xn = bf16((x - mu) * inv_sigma); 80 inputs, two hidden layers of 256.
+//
+// Tiling: one wave per group of 16 candidates (one MC state). Every layer is
+// computed transposed, out^T[hidden][cand] = W^T[hidden][k] . in^T[k][cand],
+// with v_mfma_f32_16x16x16_bf16: the weights are the A operand (lane l holds
+// row l % 16, k = 4 (l / 16) .. +3), the activations the B operand (lane l holds
+// candidate l % 16, the same k). The accumulator of a 16 x 16 output tile puts
+// rows 4 (l / 16) .. +3 of candidate l % 16 on lane l, which is exactly the B
+// operand of the next layer for the K block of that tile, so hidden
+// activations never leave the registers. Weights stream from L2 (166 KB in
+// bf16) as 8-byte loads.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ud2_internal.h"
+
+namespace ud2 {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+__device__ __forceinline__ float bf2f_s(short b) { return bf2f(uint16_t(b)); }
+
+// Round to nearest even; NaN stays NaN (quietened), overflow rounds to inf.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) {
+        if (u & 0x007FFFFFu) u |= 0x00400000u;
+        return uint16_t(u >> 16);
+    }
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Sum over the 16 lanes of a lane row (lanes 16 q .. 16 q + 15).
+__device__ __forceinline__ float row16_sum(float v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    return v;
+}
+
+// Normalised inputs of candidate `row` as B operands of the 5 K blocks of layer 1.
+__device__ __forceinline__ void load_inputs(const MlpDev& W, const uint16_t* xr, int q, bf16x4 xb[5]) {
+#pragma unroll
+    for (int ks = 0; ks < 5; ks++) {
+        const bf16x4 raw = ld4(xr + 16 * ks + 4 * q);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int f = 16 * ks + 4 * q + i;
+            xb[ks][i] = short(f2bf((bf2f_s(raw[i]) - W.mu[f]) * W.inv_sigma[f]));
+        }
+    }
+}
+
+// Layer 1: a1 tiles (hidden 16 mt + 4 q + i of candidate `row` on this lane).
+__device__ __forceinline__ void layer1(const MlpDev& W, const bf16x4 xb[5], int row, int q, bf16x4 a1[16]) {
+#pragma unroll
+    for (int mt = 0; mt < 16; mt++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* wr = W.w1t + (16 * mt + row) * kFeat + 4 * q;
+#pragma unroll
+        for (int ks = 0; ks < 5; ks++) acc = mfma16(ld4(wr + 16 * ks), xb[ks], acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) a1[mt][i] = short(f2bf(fmaxf(acc[i] + W.b1[16 * mt + 4 * q + i], 0.0f)));
+    }
+}
+
+// z2 tile ot (outputs 16 ot + 4 q + i of candidate `row`), before bias.
+__device__ __forceinline__ f32x4 layer2_tile(const MlpDev& W, const bf16x4 a1[16], int ot, int row, int q) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* wr = W.w2t + (16 * ot + row) * kHidden + 4 * q;
+#pragma unroll
+    for (int kt = 0; kt < 16; kt++) acc = mfma16(ld4(wr + 16 * kt), a1[kt], acc);
+    return acc;
+}
+
+// Inference: y[16 gi + r] for the 16 rows of group groups[gi] (or gi).
+__global__ __launch_bounds__(256) void mlp_fwd_kernel(MlpDev W, const uint16_t* __restrict__ x,
+                                                      const int32_t* __restrict__ groups, int64_t n_groups,
+                                                      float* __restrict__ y) {
+    const int64_t gi = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
+    if (gi >= n_groups) return;
+    const int64_t g = groups ? (int64_t)groups[gi] : gi;
+    const int l = threadIdx.x % kWave, row = l & 15, q = l >> 4;
+    bf16x4 xb[5];
+    load_inputs(W, x + (g * kGroup + row) * kFeat, q, xb);
+    bf16x4 a1[16];
+    layer1(W, xb, row, q, a1);
+    float yp = 0.0f;
+#pragma unroll 2
+    for (int ot = 0; ot < 16; ot++) {
+        const f32x4 acc = layer2_tile(W, a1, ot, row, q);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = 16 * ot + 4 * q + i;
+            yp += W.w3[o] * fmaxf(acc[i] + W.b2[o], 0.0f);
+        }
+    }
+    yp += __shfl_xor(yp, 16);
+    yp += __shfl_xor(yp, 32);
+    if (q == 0) y[gi * kGroup + row] = yp + W.b3[0];
+}
+
+// Backward pass of one group given dy = d loss / d y_hat: recomputes the
+// forward pass, writes the feature-major operands of the weight-gradient
+// GEMMs (xn, a1, dz1, dz2 in bf16) and the group's partial sums of dw3, db2,
+// db1 and db3.
+__global__ __launch_bounds__(256) void mlp_bwd_kernel(MlpDev W, MlpTrain T) {
+    const int gi = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (gi >= T.n_groups) return;
+    const int64_t g = T.groups ? (int64_t)T.groups[gi] : gi;
+    const int l = threadIdx.x % kWave, row = l & 15, q = l >> 4;
+    const size_t rows = (size_t)T.n_groups * kGroup;
+    const size_t col = (size_t)gi * kGroup + row;
+    float* part = T.part + (size_t)gi * kPartStride;
+    bf16x4 xb[5];
+    load_inputs(W, T.x + (g * kGroup + row) * kFeat, q, xb);
+#pragma unroll
+    for (int ks = 0; ks < 5; ks++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) T.xn_fm[(size_t)(16 * ks + 4 * q + i) * rows + col] = uint16_t(xb[ks][i]);
+    bf16x4 a1[16];
+    layer1(W, xb, row, q, a1);
+#pragma unroll
+    for (int mt = 0; mt < 16; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) T.a1_fm[(size_t)(16 * mt + 4 * q + i) * rows + col] = uint16_t(a1[mt][i]);
+    const float dy = T.dy[col];
+    bf16x4 dz2[16];
+#pragma unroll
+    for (int ot = 0; ot < 16; ot++) {
+        const f32x4 acc = layer2_tile(W, a1, ot, row, q);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = 16 * ot + 4 * q + i;
+            const float z = acc[i] + W.b2[o];
+            const float a2 = fmaxf(z, 0.0f);
+            const float d = z > 0.0f ? dy * W.w3[o] : 0.0f;
+            const uint16_t db = f2bf(d);
+            dz2[ot][i] = short(db);
+            T.dz2_fm[(size_t)o * rows + col] = db;
+            const float sw3 = row16_sum(dy * a2);
+            const float sb2 = row16_sum(d);
+            if (row == 0) {
+                part[o] = sw3;
+                part[kHidden + o] = sb2;
+            }
+        }
+    }
+    // dA1^T[h][cand] = W2[h][:] . dz2^T[:][cand]; dz1 = dA1 where a1 > 0
+#pragma unroll
+    for (int ht = 0; ht < 16; ht++) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const uint16_t* wr = W.w2 + (16 * ht + row) * kHidden + 4 * q;
+#pragma unroll
+        for (int kt = 0; kt < 16; kt++) acc = mfma16(ld4(wr + 16 * kt), dz2[kt], acc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int h = 16 * ht + 4 * q + i;
+            const float d = bf2f_s(a1[ht][i]) > 0.0f ? acc[i] : 0.0f;
+            T.dz1_fm[(size_t)h * rows + col] = f2bf(d);
+            const float sb1 = row16_sum(d);
+            if (row == 0) part[2 * kHidden + h] = sb1;
+        }
+    }
+    const float sdy = row16_sum(dy);
+    if (l == 0) part[3 * kHidden] = sdy;
+}
+
+// C[M][N] = A[M][K] . B[N][K]^T over K split into `splits` chunks: one wave per
+// 16 x 16 tile and chunk writes part[chunk][M][N]; M, N and K / splits are
+// multiples of 16.
+__global__ __launch_bounds__(256) void gemm_nt_kernel(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                                      int M, int N, int K, int splits, float* __restrict__ part) {
+    const int wid = blockIdx.x * 4 + threadIdx.x / kWave;
+    const int ntn = N / 16;
+    const int total = (M / 16) * ntn * splits;
+    if (wid >= total) return;
+    const int s = wid % splits, t = wid / splits;
+    const int tm = t / ntn, tn = t % ntn;
+    const int l = threadIdx.x % kWave, row = l & 15, q = l >> 4;
+    const int kc = K / splits, k0 = s * kc;
+    const uint16_t* ar = A + (size_t)(16 * tm + row) * K + k0 + 4 * q;
+    const uint16_t* br = B + (size_t)(16 * tn + row) * K + k0 + 4 * q;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < kc; k += 16) acc = mfma16(ld4(ar + k), ld4(br + k), acc);
+#pragma unroll
+    for (int i = 0; i < 4; i++) part[((size_t)s * M + 16 * tm + 4 * q + i) * N + 16 * tn + row] = acc[i];
+}
+
+__global__ void sum_parts_kernel(const float* __restrict__ part, int n, int splits, float* __restrict__ C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < splits; k++) s += part[(size_t)k * n + i];
+    C[i] = s;
+}
+
+// Column sums of the per-group partials into the canonical gradient slots.
+__global__ void reduce_parts_kernel(const float* __restrict__ part, int n_groups, float* __restrict__ grad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > 3 * kHidden) return;
+    float s = 0.0f;
+    for (int g = 0; g < n_groups; g++) s += part[(size_t)g * kPartStride + c];
+    int dst;
+    if (c < kHidden) dst = kOffW3 + c;
+    else if (c < 2 * kHidden) dst = kOffB2 + (c - kHidden);
+    else if (c < 3 * kHidden) dst = kOffB1 + (c - 2 * kHidden);
+    else dst = kOffB3;
+    grad[dst] = s;
+}
+
+// Group-centred loss with labels clipped at (group minimum + 6): one thread per group.
+__global__ void group_loss_kernel(const float* __restrict__ yhat, const float* __restrict__ label,
+                                  const int32_t* __restrict__ groups, int n_groups, float* __restrict__ dy,
+                                  float* __restrict__ loss_parts) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= n_groups) return;
+    const int64_t g = groups ? (int64_t)groups[gi] : gi;
+    const float* y = label + g * kGroup;
+    const float* yh = yhat + (int64_t)gi * kGroup;
+    float ymin = INFINITY;
+    for (int i = 0; i < kGroup; i++)
+        if (isfinite(y[i]) && y[i] < ymin) ymin = y[i];
+    float r[kGroup], rbar = 0.0f;
+    for (int i = 0; i < kGroup; i++) {
+        float yc = isfinite(y[i]) ? y[i] : INFINITY;
+        yc = fminf(yc, ymin + 6.0f);
+        r[i] = yh[i] - yc;
+        rbar += r[i];
+    }
+    rbar *= 1.0f / kGroup;
+    const float n = float(n_groups) * kGroup;
+    float ls = 0.0f;
+    for (int i = 0; i < kGroup; i++) {
+        const float c = isfinite(ymin) ? r[i] - rbar : 0.0f;
+        ls += c * c;
+        dy[(int64_t)gi * kGroup + i] = 2.0f * c / n;
+    }
+    loss_parts[gi] = ls;
+}
+
+__global__ void sum_loss_kernel(const float* __restrict__ parts, int n, float scale, float* __restrict__ out) {
+    __shared__ float sh[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += parts[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] * scale;
+}
+
+// Adam (Kingma & Ba, ICLR'15) with bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, int n, float lr, float beta1, float beta2, float eps, float c1,
+                            float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
+    const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 device copies of the weights from the canonical FP32 parameters.
+__global__ void mlp_pack_kernel(const float* __restrict__ P, uint16_t* __restrict__ w1t, uint16_t* __restrict__ w2t,
+                                uint16_t* __restrict__ w2, float* __restrict__ w3r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kFeat * kHidden) {
+        const int f = i / kHidden, h = i % kHidden;
+        w1t[h * kFeat + f] = f2bf(P[kOffW1 + i]);
+    }
+    if (i < kHidden * kHidden) {
+        const int h = i / kHidden, o = i % kHidden;
+        const uint16_t b = f2bf(P[kOffW2 + i]);
+        w2[i] = b;
+        w2t[o * kHidden + h] = b;
+    }
+    if (i < kHidden) w3r[i] = bf2f(f2bf(P[kOffW3 + i]));
+}
+
+// ------------------------------------------------------------- launchers
+hipError_t launch_mlp_forward(const MlpDev& W, const uint16_t* x, const int32_t* groups, int64_t n_groups, float* y,
+                              hipStream_t s) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_fwd_kernel, dim3((unsigned)((n_groups + 3) / 4)), dim3(256), 0, s, W, x, groups, n_groups,
+                       y);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_backward(const MlpDev& W, const MlpTrain& T, hipStream_t s) {
+    if (T.n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_bwd_kernel, dim3((unsigned)((T.n_groups + 3) / 4)), dim3(256), 0, s, W, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_nt(const uint16_t* A, const uint16_t* B, int M, int N, int K, int splits, float* part, float* C,
+                          hipStream_t s) {
+    if (M % 16 || N % 16 || splits <= 0 || K % (16 * splits)) return hipErrorInvalidValue;
+    const int waves = (M / 16) * (N / 16) * splits;
+    hipLaunchKernelGGL(gemm_nt_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, A, B, M, N, K, splits, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int n = M * N;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, n, splits, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_reduce_parts(const float* part, int n_groups, float* grad, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3((3 * kHidden + 1 + 255) / 256), dim3(256), 0, s, part, n_groups, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_loss(const float* yhat, const float* label, const int32_t* groups, int n_groups, float* dy,
+                             float* loss_parts, float* loss, hipStream_t s) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(group_loss_kernel, dim3((unsigned)((n_groups + 255) / 256)), dim3(256), 0, s, yhat, label,
+                       groups, n_groups, dy, loss_parts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sum_loss_kernel, dim3(1), dim3(256), 0, s, loss_parts, n_groups,
+                       1.0f / (float(n_groups) * kGroup), loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, float* m, float* v, const float* g, int n, float lr, float beta1, float beta2,
+                       float eps, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, n, lr, beta1, beta2,
+                       eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_pack(const float* params, uint16_t* w1t, uint16_t* w2t, uint16_t* w2, float* w3r,
+                           hipStream_t s) {
+    const int n = kHidden * kHidden;
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, s, params, w1t, w2t, w2, w3r);
+    return hipGetLastError();
+}
+
+}  // namespace ud2

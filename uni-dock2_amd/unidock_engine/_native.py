@@ -19,6 +19,12 @@ MODE_DIRECT = 0
 MODE_GRID = 1
 MAX_ATOMS = 256
 MAX_TORSIONS = 57
+MLP_FEATURES = 80
+MLP_HIDDEN = 256
+MLP_GROUP = 16
+MLP_NPARAM = 86785
+MLP_NNORM = 160
+POLICY_PLAIN, POLICY_EXACT, POLICY_LEARNED, POLICY_COLLECT = 0, 1, 2, 3
 
 
 class Ud2Error(RuntimeError):
@@ -40,6 +46,15 @@ class McParams(C.Structure):
     _fields_ = [("mc_steps", C.c_int32), ("opt_steps", C.c_int32), ("randomize", C.c_int32),
                 ("seed", C.c_uint32), ("mode", C.c_int32), ("lig_index_base", C.c_int32),
                 ("stats_d", C.c_void_p)]
+
+
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+class LmcParams(C.Structure):
+    _fields_ = [("mc", McParams), ("policy", C.c_int32), ("n_cand", C.c_int32), ("top_k", C.c_int32),
+                ("use_graph", C.c_int32), ("collect_x_d", C.c_void_p), ("collect_y_d", C.c_void_p)]
 
 
 _lib = None
@@ -78,6 +93,15 @@ def lib():
         "ud2_mc": (C.c_int, [vp, vp, vp, vp, vp]),
         "ud2_cluster": (C.c_int, [vp, vp, f32, vp, vp, vp]),
         "ud2_final_score": (C.c_int, [vp, vp, vp, vp, i32, vp, vp]),
+        "ud2_mlp_create": (C.c_int, [vp, vp]),
+        "ud2_mlp_set_params": (C.c_int, [vp, vp]),
+        "ud2_mlp_get_params": (C.c_int, [vp, vp]),
+        "ud2_mlp_destroy": (C.c_int, [vp]),
+        "ud2_mlp_forward": (C.c_int, [vp, vp, vp, i32, vp, vp]),
+        "ud2_mlp_backward": (C.c_int, [vp, vp, vp, i32, vp, vp, vp]),
+        "ud2_mlp_train_step": (C.c_int, [vp, vp, vp, vp, i32, vp, vp, vp]),
+        "ud2_features": (C.c_int, [vp, vp, vp, vp, i32, vp, vp]),
+        "ud2_lmc": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -92,7 +116,8 @@ EXPORTED = (
     "ud2_scene_build_grids", "ud2_scene_set_grids", "ud2_scene_grid_info", "ud2_scene_download_maps",
     "ud2_scene_destroy", "ud2_ligset_create", "ud2_ligset_destroy", "ud2_ligset_pose_layout", "ud2_init_poses",
     "ud2_score_posevecs", "ud2_energy_grad", "ud2_pose_update", "ud2_line_search", "ud2_optimize", "ud2_mc",
-    "ud2_cluster", "ud2_final_score",
+    "ud2_cluster", "ud2_final_score", "ud2_mlp_create", "ud2_mlp_set_params", "ud2_mlp_get_params",
+    "ud2_mlp_destroy", "ud2_mlp_forward", "ud2_mlp_backward", "ud2_mlp_train_step", "ud2_features", "ud2_lmc",
 )
 
 
@@ -300,3 +325,34 @@ class PoseBuffers:
 
 def version() -> str:
     return lib().ud2_version().decode()
+
+
+class Mlp:
+    """The learned proposal network on the device (canonical FP32 parameters)."""
+
+    def __init__(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        if p.size != MLP_NPARAM + MLP_NNORM:
+            raise ValueError(f"expected {MLP_NPARAM + MLP_NNORM} parameters, got {p.size}")
+        self.h = C.c_void_p()
+        check(lib().ud2_mlp_create(_ptr(p), C.byref(self.h)), "ud2_mlp_create")
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        check(lib().ud2_mlp_set_params(self.h, _ptr(p)), "ud2_mlp_set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_NPARAM + MLP_NNORM, dtype=np.float32)
+        check(lib().ud2_mlp_get_params(self.h, _ptr(out)), "ud2_mlp_get_params")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ud2_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
