@@ -1,20 +1,28 @@
 #!/usr/bin/env python
-"""Docking throughput on one MI355X: ligands/sec over the 1,024-ligand uneven batch.
+"""Docking throughput on one MI355X: ligands/sec over the 1,024-ligand uneven batch,
+learned proposal vs plain MC.
 
 Workload (synthetic, seeded): receptor of 2,000 heavy atoms in a shell around
 the pocket, 30 A box, FP32 grid maps for 8 atom types at 0.375 A (81^3 points);
-1,024 ligands with 10-60 heavy atoms and 0-15 rotatable bonds; 128 MC chains
-per ligand x 40 MC steps, each step a BFGS of (25 + natom_max) / 3 iterations;
-then clustering at 1.0 A, refinement (5 rounds x 2 BFGS iterations with the
-penalty-slope ramp), filtering (energy_range 10, num_pose 10) and final
-scoring. One "step" = one such pass over the whole batch with the inputs
-already resident on the GPU.
+1,024 ligands with 10-60 heavy atoms and 0-15 rotatable bonds. One "step" = one
+docking pass over the whole batch with the inputs already resident on the GPU:
+MC search, clustering at 1.0 A, refinement (5 rounds x 2 BFGS iterations with
+the penalty-slope ramp), filtering (energy_range 10, num_pose 10) and final
+scoring.
 
-Prints one JSON line. ``roofline`` covers the dominant kernel (the persistent
-MC kernel): achieved = grid-gather bytes of one launch (energy evaluations x
+  plain    128 chains x 40 MC steps, one random mutation + BFGS of
+           (25 + natom_max) / 3 iterations per step (the reference's search);
+  learned  the learned proposal: per step 16 mutations scored by the network
+           on MFMA, the top 4 rescored with exact Vina, the best BFGS-refined;
+           --learned-nchain chains x --learned-steps MC steps.
+
+``value`` is the learned engine's ligands/s; the plain engine is timed on the
+same batch in the same run (``config.plain``) with the median top-pose
+affinities of both. ``roofline`` covers the learned MC kernel: achieved =
+grid-gather bytes of one launch (energy evaluations and candidate features x
 32 B x heavy atoms, counted by the kernel) / its average duration measured
 with HIP events on its stream. ``cpu_baseline`` times the CPU oracle's whole
-per-ligand pipeline (direct receptor sum, as the reference scores) on a
+per-ligand plain pipeline (direct receptor sum, as the reference scores) on a
 bounded sample on the host cores.
 """
 from __future__ import annotations
@@ -47,6 +55,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["batch", "score"], default="batch",
                     help="batch: docking throughput (headline); score: E+gradient microbenchmark")
+    ap.add_argument("--search", choices=["learned", "plain"], default="learned",
+                    help="learned: headline (learned vs plain); plain: the plain engine only")
+    ap.add_argument("--learned-nchain", type=int, default=128)
+    ap.add_argument("--learned-steps", type=int, default=12)
+    ap.add_argument("--n-cand", type=int, default=16)
+    ap.add_argument("--top-k", type=int, default=4)
+    ap.add_argument("--weights", default="", help="network parameters (.npz); default: the packaged ones")
     ap.add_argument("--n-pose", type=int, default=1 << 20)
     return ap.parse_args()
 
@@ -155,6 +170,24 @@ def score_microbench(args):
     print(json.dumps(out), flush=True)
 
 
+def timed_dock(eng, ls, prm, pb, steps, warmup, tag):
+    """Warmup + timed docking passes; returns (wall seconds, results, MC launch events, kernel counters)."""
+    import torch
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    events = []
+    for i in range(warmup):
+        eng.dock(ls, prm, poses=pb)
+        print(f"# {tag} warmup step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    results = None
+    t1 = time.perf_counter()
+    for i in range(steps):
+        results, _ = eng.dock(ls, prm, poses=pb, mc_events=events, mc_stats=stats)
+        print(f"# {tag} timed step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t1, results, events, stats.cpu().numpy().astype("float64")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -164,8 +197,8 @@ def main():
     if args.workload == "score":
         return score_microbench(args)
     import numpy as np
-    import torch
     from unidock_engine import _native as N
+    from unidock_engine import proposal as LP
     from unidock_engine import synthetic as S
     from unidock_engine.engine import DockEngine, DockParams, heuristic_opt_steps
 
@@ -175,76 +208,85 @@ def main():
     box = S.docking_box()
     synth = S.config1_ligands(args.n_lig)
     ligands = [s.ligand for s in synth]
-    t_gen = time.time() - t0
-    print(f"# workload generated in {t_gen:.1f}s: {len(ligands)} ligands, "
+    print(f"# workload generated in {time.time() - t0:.1f}s: {len(ligands)} ligands, "
           f"{sum(l.natom for l in ligands)} atoms, {sum(l.npair for l in ligands)} intra pairs", file=sys.stderr,
           flush=True)
     mode = N.MODE_GRID if args.mode == "grid" else N.MODE_DIRECT
     eng = DockEngine(receptor, box, mode=mode, spacing=S.GRID_SPACING)
     ls = N.LigSet(ligands)
     opt_steps = heuristic_opt_steps(max(l.natom for l in ligands))
-    prm = DockParams(nchain=args.nchain, mc_steps=args.mc_steps, opt_steps=opt_steps, refine_steps=5, num_pose=10,
-                     rmsd_limit=1.0, energy_range=10.0, seed=12345, randomize=True, mode=mode)
-    pb = N.PoseBuffers(ls, args.nchain)
-    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
-
-    mc_times = []
-
-    for i in range(args.warmup):
-        eng.dock(ls, prm, poses=pb)
-        print(f"# warmup step {i} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    mc_times.clear()
-    stats.zero_()
-    results = None
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        results, _ = eng.dock(ls, prm, poses=pb, mc_events=mc_times, mc_stats=stats)
-        print(f"# timed step {i} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t1
-    ms_per_step = wall * 1000.0 / args.steps
-    value = len(ligands) * args.steps / wall
-
-    st = stats.cpu().numpy().astype(np.float64)
-    launches = len(mc_times)
-    mc_ms = np.asarray([a.elapsed_time(b) for a, b in mc_times])
-    # one ud2_mc call = one kernel launch per atoms-per-lane class; this batch has one class
+    base = dict(refine_steps=5, num_pose=10, rmsd_limit=1.0, energy_range=10.0, seed=12345, randomize=True,
+                mode=mode, opt_steps=opt_steps)
+    plain_prm = DockParams(nchain=args.nchain, mc_steps=args.mc_steps, search="mc", **base)
+    learned = args.search == "learned"
+    if learned:
+        eng.proposal = N.Mlp(LP.load(args.weights or LP.WEIGHTS))
+        lprm = DockParams(nchain=args.learned_nchain, mc_steps=args.learned_steps, search="learned",
+                          n_cand=args.n_cand, top_k=args.top_k, **base)
+    # plain engine: one timed pass when it is only the comparison, all passes otherwise
+    pb_plain = N.PoseBuffers(ls, args.nchain)
+    p_steps, p_warm = (1, 1) if learned else (args.steps, args.warmup)
+    p_wall, p_res, p_ev, p_st = timed_dock(eng, ls, plain_prm, pb_plain, p_steps, p_warm, "plain")
+    p_value = len(ligands) * p_steps / p_wall
+    p_top = np.asarray([float(r.energies[0][0]) if r.energies else np.nan for r in p_res])
+    if learned:
+        pb = N.PoseBuffers(ls, args.learned_nchain)
+        wall, res, ev, st = timed_dock(eng, ls, lprm, pb, args.steps, args.warmup, "learned")
+        top = np.asarray([float(r.energies[0][0]) if r.energies else np.nan for r in res])
+        nchain, msteps, kernel = args.learned_nchain, args.learned_steps, "lmc_kernel<1>"
+    else:
+        wall, res, ev, st, top = p_wall, p_res, p_ev, p_st, p_top
+        nchain, msteps, kernel = args.nchain, args.mc_steps, "mc_kernel<1>"
+    steps = args.steps
+    value = len(ligands) * steps / wall
+    mc_ms = np.asarray([a.elapsed_time(b) for a, b in ev])
+    launches = len(ev)
+    # one search call = one kernel launch per atoms-per-lane class; this batch has one class
     bytes_per_launch = st[1] / max(1, launches)
     achieved = bytes_per_launch / (float(mc_ms.mean()) / 1000.0) / 1e9 if launches else 0.0
-    top = [float(r.energies[0][0]) for r in results if r.energies]
-
+    ok = np.isfinite(top) & np.isfinite(p_top)
+    cfg = {
+        "workload": f"{len(ligands)}-ligand uneven batch (10-60 heavy atoms, 0-15 rotatable bonds), MC search + "
+                    f"cluster + refine + score, {args.mode} inter term",
+        "method": (f"learned proposal: {nchain} chains x {msteps} MC steps, {args.n_cand} candidates per step "
+                   f"scored by the network, top {args.top_k} rescored exactly, best BFGS {opt_steps}"
+                   if learned else f"plain MC: {nchain} chains x {msteps} MC steps x BFGS {opt_steps}"),
+        "n_ligands": len(ligands),
+        "nchain": nchain,
+        "mc_steps": msteps,
+        "opt_steps": opt_steps,
+        "inter_mode": args.mode,
+        "mc_ms_per_launch": round(float(mc_ms.mean()), 3) if launches else None,
+        "energy_evals_per_step": int(st[0] / steps),
+        "evals_per_sec": round(st[0] / wall, 1),
+        "median_top_affinity": round(float(np.nanmedian(top)), 4),
+    }
+    if learned:
+        cfg["plain"] = {
+            "value": round(p_value, 4), "unit": "ligands/s", "nchain": args.nchain, "mc_steps": args.mc_steps,
+            "ms_per_step": round(p_wall * 1000.0 / p_steps, 3),
+            "median_top_affinity": round(float(np.nanmedian(p_top)), 4),
+        }
+        cfg["speedup_vs_plain"] = round(value / p_value, 3)
+        cfg["median_top_affinity_diff"] = round(float(np.nanmedian(top) - np.nanmedian(p_top)), 4)
+        cfg["median_per_ligand_top_diff"] = round(float(np.median(top[ok] - p_top[ok])), 4)
+        cfg["frac_ligands_within_0.1"] = round(float(np.mean(top[ok] - p_top[ok] <= 0.1)), 4)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
         "unit": "ligands/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(wall * 1000.0 / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if not learned else "fp32 (network: bf16 x bf16 -> fp32 MFMA)",
         "data": "synthetic (seeded receptor shell and random torsion-tree ligands)",
-        "config": {
-            "workload": f"{len(ligands)}-ligand uneven batch (10-60 heavy atoms, 0-15 rotatable bonds), "
-                        f"{args.nchain} chains x {args.mc_steps} MC steps x BFGS {opt_steps}, cluster + refine + "
-                        f"score, {args.mode} inter term",
-            "method": "plain MC (learned proposal not enabled)",
-            "n_ligands": len(ligands),
-            "nchain": args.nchain,
-            "mc_steps": args.mc_steps,
-            "opt_steps": opt_steps,
-            "inter_mode": args.mode,
-            "mc_ms_per_launch": round(float(mc_ms.mean()), 3) if launches else None,
-            "energy_evals_per_step": int(st[0] / args.steps),
-            "mc_steps_per_sec": round(len(ligands) * args.nchain * args.mc_steps * args.steps / wall, 1),
-            "evals_per_sec": round(st[0] / wall, 1),
-            "median_top_affinity": round(float(np.median(top)), 4) if top else None,
-        },
+        "config": cfg,
         "roofline": {
-            "kernel": "mc_kernel<1>",
+            "kernel": kernel,
             "bound": "hbm",
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,

@@ -191,7 +191,6 @@ typedef struct {
     int32_t policy;       /* UD2_POLICY_* */
     int32_t n_cand;       /* mutations per step, 1..16 (ignored by the plain policy) */
     int32_t top_k;        /* learned policy: candidates rescored exactly, 1..n_cand */
-    int32_t use_graph;    /* capture the rounds of the search into one HIP graph */
     uint16_t* collect_x_d;  /* collect: [mc_steps][n_pose][16][80] features (bf16) */
     float* collect_y_d;     /* collect: [mc_steps][n_pose][16] energies after BFGS */
 } ud2_lmc_params;
@@ -216,8 +215,8 @@ int ud2_mlp_train_step(ud2_mlp* m, const uint16_t* x_d, const float* label_d, co
 int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
                  uint16_t* x_d, void* stream);
 /* Learned-proposal Monte Carlo search; best pose per chain written to P as
- * ud2_mc does. mlp may be NULL except for the learned policy. Returns after
- * the search has finished. */
+ * ud2_mc does (one persistent launch per atoms-per-lane class over a
+ * size-sorted work queue). mlp may be NULL except for the learned policy. */
 int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_lmc_params* params,
             const ud2_mlp* mlp, void* stream);
 

@@ -988,8 +988,8 @@ int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, 
     return UD2_OK;
 }
 
-// Learned-proposal MC: state buffers of the chains, per-class work orders and
-// one launch sequence per round, optionally captured into a HIP graph.
+// Learned-proposal MC: one persistent launch per atoms-per-lane class over a
+// size-sorted queue, as ud2_mc; the chains keep their state on chip.
 int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const ud2_lmc_params* prm,
             const ud2_mlp* mlp, void* stream) {
     int rc = check_poses(ls, P);
@@ -1014,37 +1014,27 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
     if (policy == UD2_POLICY_COLLECT && (!prm->collect_x_d || !prm->collect_y_d))
         return fail(UD2_ERR_ARG, "collect policy without output buffers");
     const int64_t npose = int64_t(ls->n_lig) * P->nchain;
-    if (npose > (int64_t(1) << 26)) return fail(UD2_ERR_LIMIT, "too many chains");
-    hipStream_t caller = static_cast<hipStream_t>(stream);
-
-    // chain state: accepted pose, Philox positions, candidate features / scores
-    int64_t nx = 0, nd = 0, np = 0;
-    rc = ud2_ligset_pose_layout(ls, P->nchain, &nx, &nd, &np);
-    if (rc) return rc;
-    std::vector<void*> owned;
-    auto dev = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
-        owned.push_back(p);
-        return p;
-    };
-    struct Cleanup {
-        std::vector<void*>* v;
-        ~Cleanup() { for (void* p : *v) (void)hipFree(p); }
-    } cleanup{&owned};
-    PoseBufs acc;
-    acc.nchain = P->nchain;
-    acc.x = static_cast<float*>(dev(size_t(nx) * sizeof(float)));
-    acc.dihe = static_cast<float*>(dev(size_t(nd) * sizeof(float)));
-    acc.c = static_cast<float*>(dev(size_t(np) * 3 * sizeof(float)));
-    acc.rv = static_cast<float*>(dev(size_t(np) * 3 * sizeof(float)));
-    acc.e = static_cast<float*>(dev(size_t(np) * sizeof(float)));
-    uint32_t* draw = static_cast<uint32_t*>(dev(size_t(np) * sizeof(uint32_t)));
-    uint32_t* cand_draw = static_cast<uint32_t*>(dev(size_t(np) * kGroup * sizeof(uint32_t)));
-    uint16_t* xs = (feats && policy != UD2_POLICY_COLLECT)
-                       ? static_cast<uint16_t*>(dev(size_t(np) * kGroup * kFeat * sizeof(uint16_t)))
-                       : nullptr;
-    float* pred = policy == UD2_POLICY_LEARNED ? static_cast<float*>(dev(size_t(np) * kGroup * sizeof(float))) : nullptr;
+    if (npose > (int64_t(1) << 30)) return fail(UD2_ERR_LIMIT, "too many chains");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    LmcArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.L = ls->view();
+    A.S = sc->view();
+    A.P = pose_view(P);
+    A.mode = mc.mode;
+    A.mc_steps = mc.mc_steps;
+    A.opt_steps = mc.opt_steps;
+    A.randomize = mc.randomize;
+    A.seed = mc.seed;
+    A.lig_index_base = mc.lig_index_base;
+    A.policy = policy;
+    A.n_cand = n_cand;
+    A.top_k = prm->top_k;
+    if (mlp) A.W = mlp->view();
+    A.n_pose = npose;
+    A.collect_x = prm->collect_x_d;
+    A.collect_y = prm->collect_y_d;
+    A.stats = reinterpret_cast<unsigned long long*>(mc.stats_d);
     std::vector<int32_t> order(ls->n_lig);
     std::iota(order.begin(), order.end(), 0);
     auto cost = [&](int l) {
@@ -1055,130 +1045,41 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
         if (ls->h_ligs[a].apl != ls->h_ligs[b].apl) return ls->h_ligs[a].apl > ls->h_ligs[b].apl;
         return cost(a) > cost(b);
     });
-    // one queue counter per (round, kernel, class), all cleared once up front
-    const int steps = mc.mc_steps;
-    const int n_counters = (1 + 2 * steps) * (kMaxApl + 1);
-    int32_t* counters = static_cast<int32_t*>(dev(size_t(n_counters) * sizeof(int32_t)));
-    int32_t* d_order = static_cast<int32_t*>(dev(order.size() * sizeof(int32_t)));
-    if (!acc.x || !acc.dihe || !acc.c || !acc.rv || !acc.e || !draw || !cand_draw || !counters || !d_order ||
-        (feats && policy != UD2_POLICY_COLLECT && !xs) || (policy == UD2_POLICY_LEARNED && !pred))
-        return fail(UD2_ERR_HIP, "hipMalloc of the chain state failed");
-    UD2_HIP(hipMemcpy(d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-
-    struct Cls {
-        int apl, pos, cnt, dmax;
-    };
-    std::vector<Cls> classes;
-    for (int pos = 0; pos < ls->n_lig;) {
-        Cls c{int(ls->h_ligs[order[pos]].apl), pos, 0, 6};
-        while (pos < ls->n_lig && int(ls->h_ligs[order[pos]].apl) == c.apl) {
-            c.dmax = std::max(c.dmax, 6 + int(ls->h_ligs[order[pos]].ntors));
-            c.cnt++;
-            pos++;
-        }
-        classes.push_back(c);
-    }
+    UD2_HIP(hipMemcpyAsync(ls->d_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    UD2_HIP(launch_fill_i32(ls->d_counter, 0, kMaxApl + 1, st));
+    int pos = 0;
     const int cus = n_cus();
-    LmcArgs A;
-    A.L = ls->view();
-    A.S = sc->view();
-    A.best = pose_view(P);
-    A.acc = acc;
-    A.mode = mc.mode;
-    A.opt_steps = mc.opt_steps;
-    A.randomize = mc.randomize;
-    A.seed = mc.seed;
-    A.lig_index_base = mc.lig_index_base;
-    A.policy = policy;
-    A.n_cand = n_cand;
-    A.top_k = prm->top_k;
-    A.draw = draw;
-    A.cand_draw = cand_draw;
-    A.pred = pred;
-    A.label = nullptr;
-    A.stats = reinterpret_cast<unsigned long long*>(mc.stats_d);
-
-    // launch sequence; counters are indexed (slot * (kMaxApl + 1) + apl)
-    auto enqueue = [&](hipStream_t st) -> int {
-        UD2_HIP(launch_fill_i32(counters, 0, n_counters, st));
-        auto run_kernel = [&](int kernel, int slot) -> int {
-            for (const Cls& c : classes) {
-                LmcArgs a = A;
-                a.order = d_order + c.pos;
-                a.n_order = c.cnt;
-                a.counter = counters + slot * (kMaxApl + 1) + c.apl;
-                a.hdim_ld = odd_ld(c.dmax);
-                a.lds_floats_per_wave = kernel == 1 ? lds_prop_floats(c.apl)
-                                                    : lds_fixed_floats(c.apl) + c.dmax * a.hdim_ld;
-                const int lds_bytes = a.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
-                if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
-                const int64_t items = int64_t(c.cnt) * P->nchain;
-                const int64_t need = (items + kWavesPerBlock - 1) / kWavesPerBlock;
-                const int64_t cap = int64_t(cus) * lmc_blocks_per_cu(kernel, c.apl, lds_bytes);
-                const int nb = int(std::max<int64_t>(1, std::min(need, cap)));
-                hipError_t e = kernel == 0 ? launch_lmc_init(a, c.apl, nb, st)
-                             : kernel == 1 ? launch_lmc_propose(a, c.apl, nb, st)
-                                           : launch_lmc_step(a, c.apl, nb, st);
-                if (e != hipSuccess) return fail(UD2_ERR_HIP, std::string("lmc launch: ") + hipGetErrorString(e));
-            }
-            return UD2_OK;
-        };
-        int r = run_kernel(0, 0);
-        if (r) return r;
-        for (int s = 0; s < steps; s++) {
-            A.step = s;
-            if (policy == UD2_POLICY_COLLECT) {
-                A.x = prm->collect_x_d + size_t(s) * size_t(np) * kGroup * kFeat;
-                A.label = prm->collect_y_d + size_t(s) * size_t(np) * kGroup;
-            } else {
-                A.x = xs;
-            }
-            if (feats) {
-                r = run_kernel(1, 1 + 2 * s);
-                if (r) return r;
-            }
-            if (policy == UD2_POLICY_LEARNED)
-                UD2_HIP(launch_mlp_forward(mlp->view(), xs, nullptr, np, pred, st));
-            r = run_kernel(2, 2 + 2 * s);
-            if (r) return r;
+    for (int apl = kMaxApl; apl >= 1; apl--) {
+        int cnt = 0, dmax = 6;
+        for (int k = pos; k < ls->n_lig && ls->h_ligs[order[k]].apl == apl; k++) {
+            cnt++;
+            dmax = std::max(dmax, 6 + int(ls->h_ligs[order[k]].ntors));
         }
-        return UD2_OK;
-    };
-
-    if (!prm->use_graph) {
-        rc = enqueue(caller);
-        if (rc) return rc;
-        // the chain state is freed on return: finish the search first
-        UD2_HIP(hipStreamSynchronize(caller));
-        return UD2_OK;
+        if (cnt == 0) continue;
+        A.order = ls->d_order + pos;
+        A.n_order = cnt;
+        A.counter = ls->d_counter + apl;
+        A.hdim_ld = odd_ld(dmax);
+        // While candidates are scored the energy's LDS is idle: the 16 bf16
+        // candidate rows (640 floats) take the coordinate / force / BFGS-start
+        // area at the slice start, the feature scratch (8 floats per atom) and
+        // the hidden layer (16 x 264 bf16) the Hessian's. Offsets stay even.
+        const int fixed = lds_fixed_floats(apl);
+        const int a1_floats = kGroup * (kHidden + 8) / 2;
+        const int shared = feats ? std::max(dmax * A.hdim_ld, std::max(8 * kWave * apl, a1_floats)) : dmax * A.hdim_ld;
+        static_assert(kGroup * kFeat / 2 <= 7 * kWave + kWave * 4 + 8, "candidate rows exceed the idle LDS");
+        A.lfeat = 0;
+        A.lscr = fixed;
+        A.la1 = fixed;
+        A.lds_floats_per_wave = (fixed + shared + 1) & ~1;
+        const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
+        if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
+        const int64_t items = int64_t(cnt) * P->nchain;
+        const int64_t need = (items + kWavesPerBlock - 1) / kWavesPerBlock;
+        const int64_t cap = int64_t(cus) * lmc_blocks_per_cu(apl, lds_bytes);
+        const int nb = int(std::max<int64_t>(1, std::min(need, cap)));
+        UD2_HIP(launch_lmc(A, apl, nb, st));
+        pos += cnt;
     }
-    // graph: captured on a private stream (default priority) ordered after the caller's stream
-    hipStream_t cs = nullptr;
-    UD2_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    hipEvent_t ev = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    auto done = [&](int code) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (ev) (void)hipEventDestroy(ev);
-        if (cs) (void)hipStreamDestroy(cs);
-        return code;
-    };
-    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return done(fail(UD2_ERR_HIP, "event"));
-    if (hipEventRecord(ev, caller) != hipSuccess || hipStreamWaitEvent(cs, ev, 0) != hipSuccess)
-        return done(fail(UD2_ERR_HIP, "stream ordering"));
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
-        return done(fail(UD2_ERR_HIP, "hipStreamBeginCapture failed"));
-    rc = enqueue(cs);
-    hipGraph_t g = nullptr;
-    hipError_t ec = hipStreamEndCapture(cs, &g);
-    graph = g;
-    if (rc) return done(rc);
-    if (ec != hipSuccess) return done(fail(UD2_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec)));
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
-        return done(fail(UD2_ERR_HIP, "hipGraphInstantiate failed"));
-    if (hipGraphLaunch(exec, cs) != hipSuccess) return done(fail(UD2_ERR_HIP, "hipGraphLaunch failed"));
-    if (hipStreamSynchronize(cs) != hipSuccess) return done(fail(UD2_ERR_HIP, "graph execution failed"));
-    return done(UD2_OK);
+    return UD2_OK;
 }

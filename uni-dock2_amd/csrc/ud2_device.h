@@ -672,9 +672,12 @@ __device__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_ca
     save_pose(C, x);
     float p = g;
     for (int step = 0; step < max_steps; step++) {
+        // readlane stays in wave-uniform control flow; rows beyond dim are masked
         float s = 0.0f;
-        if (row)
-            for (int j = 0; j < dim; j++) s += s_lds[hrow + j] * rdl(g, j);
+        for (int j = 0; j < dim; j++) {
+            const float gj = rdl(g, j);
+            if (row) s += s_lds[hrow + j] * gj;
+        }
         p = row ? -s : 0.0f;
         line_search(C, x, g, p, E, v_cap, slope, x_new, g_new, E1, alpha);
         const float y = g_new - g;
@@ -693,17 +696,18 @@ __device__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_ca
         const float yp = uni(wave_sum(y * p));
         if (alpha * yp < kEps) continue;
         float hy = 0.0f;
-        if (row)
-            for (int j = 0; j < dim; j++) hy += s_lds[hrow + j] * rdl(y, j);
+        for (int j = 0; j < dim; j++) {
+            const float yj = rdl(y, j);
+            if (row) hy += s_lds[hrow + j] * yj;
+        }
         const float mhy = row ? -hy : 0.0f;
         const float yhy = -wave_sum(y * mhy);
         const float r = 1.0f / (alpha * yp);
         const float a1 = uni(alpha * r), a2 = uni(alpha * alpha * (r * r * yhy + r));
-        if (row)
-            for (int j = 0; j < dim; j++) {
-                const float pj = rdl(p, j), mj = rdl(mhy, j);
-                s_lds[hrow + j] += a1 * (mhy * pj + mj * p) + a2 * p * pj;
-            }
+        for (int j = 0; j < dim; j++) {
+            const float pj = rdl(p, j), mj = rdl(mhy, j);
+            if (row) s_lds[hrow + j] += a1 * (mhy * pj + mj * p) + a2 * p * pj;
+        }
     }
     if (E > E_ori) {
         restore_pose(C, x);

@@ -179,27 +179,26 @@ constexpr int kPartStride = 3 * kHidden + 4;
 
 enum LmcPolicy : int32_t { kPolicyPlain = 0, kPolicyExact = 1, kPolicyLearned = 2, kPolicyCollect = 3 };
 
-// One round of the learned-proposal MC over every chain of a launch class.
+// Persistent learned-proposal MC over the chains of one launch class.
 struct LmcArgs {
     LigSetDev L;
     SceneDev S;
-    PoseBufs best;           // best pose of each chain (the result)
-    PoseBufs acc;            // accepted pose of each chain
+    PoseBufs P;              // best pose of each chain (the result)
     const int32_t* order;
     int32_t n_order;
-    int32_t* counter;        // work-queue head of this launch, zero at launch
+    int32_t* counter;        // work-queue head, zero at launch
     int32_t mode;            // inter term of the search energy
-    int32_t opt_steps, randomize;
+    int32_t mc_steps, opt_steps, randomize;
     uint32_t seed;
     int32_t lig_index_base;
     int32_t lds_floats_per_wave;
     int32_t hdim_ld;
-    int32_t policy, n_cand, top_k, step;
-    uint32_t* draw;          // [n_pose] next Philox block of each chain
-    uint32_t* cand_draw;     // [n_pose][16] first block of each candidate's mutation
-    uint16_t* x;             // [n_pose][16][80] candidate features
-    const float* pred;       // [n_pose][16] network scores
-    float* label;            // collect: [n_pose][16] energies after BFGS
+    int32_t lscr, lfeat, la1;  // feature scratch, candidate rows, hidden layer (floats from the slice start)
+    int32_t policy, n_cand, top_k;
+    MlpDev W;
+    int64_t n_pose;
+    uint16_t* collect_x;     // collect: [mc_steps][n_pose][16][80] features
+    float* collect_y;        // collect: [mc_steps][n_pose][16] energies after BFGS
     unsigned long long* stats;
 };
 
@@ -230,11 +229,9 @@ int posevec_blocks_per_cu(int apl, int lds_bytes_per_block);
 __host__ __device__ constexpr int lds_prop_floats(int apl) { return lds_fixed_floats(apl) + 8 * kWave * apl; }
 
 // Learned proposal (ud2_learned.hip) and network (ud2_mlp.hip) launchers.
-hipError_t launch_lmc_init(LmcArgs a, int apl, int n_blocks, hipStream_t s);
-hipError_t launch_lmc_propose(LmcArgs a, int apl, int n_blocks, hipStream_t s);
-hipError_t launch_lmc_step(LmcArgs a, int apl, int n_blocks, hipStream_t s);
+hipError_t launch_lmc(LmcArgs a, int apl, int n_blocks, hipStream_t s);
 hipError_t launch_features(FeatArgs a, int apl, hipStream_t s);
-int lmc_blocks_per_cu(int kernel, int apl, int lds_bytes_per_block);
+int lmc_blocks_per_cu(int apl, int lds_bytes_per_block);
 hipError_t launch_mlp_forward(const MlpDev& W, const uint16_t* x, const int32_t* groups, int64_t n_groups, float* y,
                               hipStream_t s);
 hipError_t launch_mlp_backward(const MlpDev& W, const MlpTrain& T, hipStream_t s);

@@ -1,37 +1,32 @@
-// Round kernels of the learned-proposal Monte Carlo search (no reference
-// counterpart; the plain search is mc_kernel, after E/src/cuda/mc.cu:275-398).
+// Learned-proposal Monte Carlo search (no reference counterpart; the plain
+// search is mc_kernel, after E/src/cuda/mc.cu:275-398).
 //
-// One MC step of every chain is one round of launches:
-//   lmc_propose_kernel  16 candidate mutations of the accepted pose, each
-//                       described by 80 pooled grid-map features (bf16);
-//   mlp_fwd_kernel      the network scores the 16 candidates of every chain;
-//   lmc_step_kernel     the top_k candidates by score are rebuilt and scored
-//                       with the exact search energy, the best is BFGS-refined
-//                       and goes through the reference's Metropolis test.
-// Policies: plain (one random candidate, no network: the reference's step),
-// exact (n_cand candidates all scored exactly, no network), learned (above),
-// collect (every candidate BFGS-refined; its energy is the training label;
-// the chain moves on with candidate 0, i.e. the plain trajectory).
-// A candidate is identified by the Philox block where its mutation starts, so
-// rebuilding it replays the same draws. Every chain's state (accepted pose,
-// best pose, next Philox block) lives in HBM between rounds, so the rounds of
-// a search can be captured in one HIP graph.
+// lmc_kernel is persistent like mc_kernel: each wave takes (ligand, chain)
+// items from a size-sorted queue and runs the whole chain with its state in
+// registers and LDS. One MC step, by policy:
+//   plain    one random mutation (the reference's step);
+//   exact    n_cand mutations all scored with the exact search energy, the
+//            best one kept;
+//   learned  n_cand mutations described by 80 pooled grid-map features each
+//            (bf16 rows in LDS), scored together by the network on MFMA
+//            (one 16-candidate group per wave), the top_k rescored exactly and
+//            the best kept;
+//   collect  n_cand mutations whose features and BFGS-refined energies are
+//            written out as training data; the chain continues with
+//            candidate 0, i.e. the plain trajectory.
+// The kept candidate is BFGS-refined and goes through the reference's
+// Metropolis test (mc.cu:340-392). A candidate is identified by the Philox
+// block where its mutation starts, so rebuilding it replays the same draws.
+// While candidates are drawn and scored, the LDS that the energy and BFGS use
+// is idle: the candidate rows take the coordinate / force / BFGS-start area
+// and the feature scratch and the network's hidden layer take the Hessian's.
 #include "ud2_device.h"
+#include "ud2_mlp.h"
 
 namespace ud2 {
 
 // Feature class of each XS type (E/src/constants/constants.h:48-68); -1 = hydrogen.
 static __constant__ int c_fclass[kNumTypes] = {-1, 1, 0, 1, 2, 3, 4, 3, 5, 6, 5, 6, 0, 1, 7, 7, 0, 0, 0, 0, 7};
-
-__device__ __forceinline__ uint16_t f2bf_l(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) {
-        if (u & 0x007FFFFFu) u |= 0x00400000u;
-        return uint16_t(u >> 16);
-    }
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
 
 // Features of pose P (80 bf16 into xrow). Per heavy atom: its own map's value e
 // and gradient g at its position, statistics [1, min(e,0), clamp(e,0,1),
@@ -83,7 +78,7 @@ __device__ void pose_features(const Ctx<APL>& C, const WPose<APL>& P, const Move
         const float v = st == 0 ? 1.0f : s_lds[sc + 8 * a + st];
         f += c == my_cls ? v : 0.0f;
     }
-    xrow[ln] = f2bf_l(f);
+    xrow[ln] = f2bf(f);
     curl = wave_sum(curl);
     if (ln < 16) {
         float g = 0.0f;
@@ -96,171 +91,147 @@ __device__ void pose_features(const Ctx<APL>& C, const WPose<APL>& P, const Move
         if (ln == 6) g = fminf(fmaxf(e_acc, -20.0f), 20.0f) / 10.0f;
         if (ln == 7) g = mv.moved;
         if (ln == 8) g = fminf(fmaxf(curl, -20.0f), 20.0f) / 10.0f;
-        xrow[kPooled + ln] = f2bf_l(g);
+        xrow[kPooled + ln] = f2bf(g);
     }
     wave_lds_fence();
 }
 
-// Round 0: input pose, randomisation (E/src/cuda/mc.cu:109-134); the accepted
-// pose starts at energy 999 as the reference's does, the best at 1e9.
+// Persistent learned-proposal MC kernel.
 template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_init_kernel(LmcArgs A) {
+__global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
     const int w = threadIdx.x / kWave;
     const int base = w * A.lds_floats_per_wave;
-    const int nchain = A.acc.nchain;
+    const int nchain = A.P.nchain;
     const int n_items = A.n_order * nchain;
+    const int ln = lane();
+    uint16_t* xl = reinterpret_cast<uint16_t*>(s_lds + base + A.lfeat);
+    uint16_t* a1l = reinterpret_cast<uint16_t*>(s_lds + base + A.la1);
     for (;;) {
         int item = 0;
-        if (lane() == 0) item = atomicAdd(A.counter, 1);
+        if (ln == 0) item = atomicAdd(A.counter, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const int lig = A.order[item / nchain];
         const int chain = item % nchain;
+        const size_t pid = (size_t)lig * nchain + chain;
         Ctx<APL> C;
         load_ctx(C, A.L, A.S, lig, A.mode, base, A.hdim_ld);
         Rng rng{A.seed, uint32_t(lig + A.lig_index_base), uint32_t(chain), 0u};
-        WPose<APL> p;
-        initial_pose(C, A.L, lig, p);
-        if (A.randomize) randomize(C, p, rng, chain, nchain);
-        store_pose(A.L, A.acc, lig, chain, p);
-        p.e = 1e9f;
-        store_pose(A.L, A.best, lig, chain, p);
-        if (lane() == 0) A.draw[(size_t)lig * nchain + chain] = rng.draw;
-    }
-}
-
-// Candidate mutations of the accepted pose and their features.
-template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_propose_kernel(LmcArgs A) {
-    const int w = threadIdx.x / kWave;
-    const int base = w * A.lds_floats_per_wave;
-    const int nchain = A.acc.nchain;
-    const int n_items = A.n_order * nchain;
-    const int ln = lane();
-    for (;;) {
-        int item = 0;
-        if (ln == 0) item = atomicAdd(A.counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= n_items) break;
-        const int lig = A.order[item / nchain];
-        const int chain = item % nchain;
-        const size_t pid = (size_t)lig * nchain + chain;
-        Ctx<APL> C;
-        load_ctx(C, A.L, A.S, lig, A.mode, base, A.hdim_ld);
         WPose<APL> acc;
-        load_pose<APL>(A.L, A.acc, lig, chain, acc);
-        save_pose(C, acc, 1);
-        Rng rng{A.seed, uint32_t(lig + A.lig_index_base), uint32_t(chain), A.draw[pid]};
-        uint32_t my_start = 0;
-        for (int c = 0; c < A.n_cand; c++) {
-            if (ln == c) my_start = rng.draw;
+        initial_pose(C, A.L, lig, acc);
+        if (A.randomize) randomize(C, acc, rng, chain, nchain);
+        store_pose(A.L, A.P, lig, chain, acc);
+        save_pose(C, acc, 1);  // the accepted pose lives in LDS between steps
+        float best = 1e9f;
+        C.evals = 0;
+        int n_feat = 0;
+        for (int step = 0; step < A.mc_steps; step++) {
             WPose<APL> cand;
             restore_pose(C, cand, 1);
-            const Move mv = mutate(C, cand, rng);
-            pose_features(C, cand, mv, acc.e, C.lh, A.x + (pid * kGroup + c) * kFeat);
-        }
-        if (ln < A.n_cand) A.cand_draw[pid * kGroup + ln] = my_start;
-        if (ln == 0) A.draw[pid] = rng.draw;
-        if (A.stats && ln == 0)
-            atomicAdd(&A.stats[1], (unsigned long long)A.n_cand * 32ull * (unsigned long long)C.nheavy);
-    }
-}
-
-// Selection, BFGS and the Metropolis step (mc.cu:340-392) of one round.
-template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_step_kernel(LmcArgs A) {
-    const int w = threadIdx.x / kWave;
-    const int base = w * A.lds_floats_per_wave;
-    const int nchain = A.acc.nchain;
-    const int n_items = A.n_order * nchain;
-    const int ln = lane();
-    for (;;) {
-        int item = 0;
-        if (ln == 0) item = atomicAdd(A.counter, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= n_items) break;
-        const int lig = A.order[item / nchain];
-        const int chain = item % nchain;
-        const size_t pid = (size_t)lig * nchain + chain;
-        Ctx<APL> C;
-        load_ctx(C, A.L, A.S, lig, A.mode, base, A.hdim_ld);
-        WPose<APL> acc;
-        load_pose<APL>(A.L, A.acc, lig, chain, acc);
-        save_pose(C, acc, 1);
-        const float e_acc = acc.e;
-        float best_e = uni(A.best.e[pid]);
-        Rng rng{A.seed, uint32_t(lig + A.lig_index_base), uint32_t(chain), A.draw[pid]};
-        WPose<APL> cand = acc;
-        if (A.policy == kPolicyPlain) {
-            mutate(C, cand, rng);
-            bfgs(C, cand, A.opt_steps, kVCap, kSlope);
-        } else if (A.policy == kPolicyCollect) {
-            // every candidate refined; candidate 0 (processed last) continues the chain
-            for (int c = A.n_cand - 1; c >= 0; c--) {
-                Rng r2 = rng;
-                r2.draw = A.cand_draw[pid * kGroup + c];
+            const float acc_e = cand.e;
+            const size_t crow = ((size_t)step * A.n_pose + pid) * kGroup;
+            if (A.policy == kPolicyPlain) {
+                mutate(C, cand, rng);
+            } else if (A.policy == kPolicyExact) {
+                float be = INFINITY;
+                uint32_t bs = rng.draw;
+                for (int c = 0; c < A.n_cand; c++) {
+                    const uint32_t start = rng.draw;
+                    WPose<APL> t;
+                    restore_pose(C, t, 1);
+                    mutate(C, t, rng);
+                    Forces<APL> F;
+                    const float e = energy_forces(C, t, kVCap, kSlope, F);
+                    if (e < be || c == 0) { be = e; bs = start; }
+                }
+                Rng r3 = rng;
+                r3.draw = bs;
                 restore_pose(C, cand, 1);
-                mutate(C, cand, r2);
-                bfgs(C, cand, A.opt_steps, kVCap, kSlope);
-                if (ln == 0) A.label[pid * kGroup + c] = cand.e;
-            }
-        } else {
-            // exact: all n_cand drawn here in order; learned: the top_k by score, rebuilt
-            int rank = 0;
-            if (A.policy == kPolicyLearned) {
-                float p = ln < A.n_cand ? A.pred[pid * kGroup + ln] : INFINITY;
-                p = p == p ? p : INFINITY;
-                for (int j = 0; j < A.n_cand; j++) {
-                    const float pj = rdl(p, j);
-                    rank += (pj < p || (pj == p && j < ln)) ? 1 : 0;
+                mutate(C, cand, r3);
+            } else {
+                // candidate c starts at Philox block my_start of lane c
+                uint32_t my_start = 0;
+                for (int c = 0; c < A.n_cand; c++) {
+                    if (ln == c) my_start = rng.draw;
+                    WPose<APL> t;
+                    restore_pose(C, t, 1);
+                    const Move mv = mutate(C, t, rng);
+                    uint16_t* dst = A.policy == kPolicyCollect ? A.collect_x + (crow + c) * kFeat : xl + c * kFeat;
+                    pose_features(C, t, mv, acc_e, base + A.lscr, dst);
                 }
-            }
-            // candidates are scored one at a time and the winner rebuilt from its
-            // first Philox block, so only one candidate pose is live at a time
-            const int n_try = A.policy == kPolicyLearned ? A.top_k : A.n_cand;
-            float be = INFINITY;
-            uint32_t best_start = rng.draw;
-            for (int r = 0; r < n_try; r++) {
-                uint32_t start = rng.draw;
+                n_feat += A.n_cand;
                 if (A.policy == kPolicyLearned) {
-                    const unsigned long long m = __ballot(ln < A.n_cand && rank == r);
-                    start = A.cand_draw[pid * kGroup + (__ffsll((long long)m) - 1)];
+                    wave_lds_fence();
+                    float p = mlp_group_scores_lds(A.W, xl, a1l);  // lane 16 q + r: score of candidate r
+                    p = (ln < A.n_cand && p == p) ? p : INFINITY;
+                    int rank = 0;
+                    for (int j = 0; j < A.n_cand; j++) {
+                        const float pj = rdl(p, j);
+                        rank += (pj < p || (pj == p && j < ln)) ? 1 : 0;
+                    }
+                    float be = INFINITY;
+                    uint32_t bs = 0;
+                    for (int r = 0; r < A.top_k; r++) {
+                        const unsigned long long m = __ballot(ln < A.n_cand && rank == r);
+                        const int c = __ffsll((long long)m) - 1;
+                        const uint32_t start = uint32_t(rdl_i(int(my_start), c));
+                        Rng r2 = rng;
+                        r2.draw = start;
+                        WPose<APL> t;
+                        restore_pose(C, t, 1);
+                        mutate(C, t, r2);
+                        Forces<APL> F;
+                        const float e = energy_forces(C, t, kVCap, kSlope, F);
+                        if (e < be || r == 0) { be = e; bs = start; }
+                    }
+                    Rng r3 = rng;
+                    r3.draw = bs;
+                    restore_pose(C, cand, 1);
+                    mutate(C, cand, r3);
+                } else {
+                    // collect: every other candidate refined for its label
+                    for (int c = A.n_cand - 1; c >= 1; c--) {
+                        Rng r2 = rng;
+                        r2.draw = uint32_t(rdl_i(int(my_start), c));
+                        WPose<APL> t;
+                        restore_pose(C, t, 1);
+                        mutate(C, t, r2);
+                        bfgs(C, t, A.opt_steps, kVCap, kSlope);
+                        if (ln == 0) A.collect_y[crow + c] = t.e;
+                    }
+                    Rng r2 = rng;
+                    r2.draw = uint32_t(rdl_i(int(my_start), 0));
+                    restore_pose(C, cand, 1);
+                    mutate(C, cand, r2);
                 }
-                Rng r2 = rng;
-                r2.draw = start;
-                WPose<APL> t;
-                restore_pose(C, t, 1);
-                mutate(C, t, r2);
-                if (A.policy != kPolicyLearned) rng.draw = r2.draw;
-                Forces<APL> F;
-                const float e = energy_forces(C, t, kVCap, kSlope, F);
-                if (e < be || r == 0) { be = e; best_start = start; }
             }
-            Rng r3 = rng;
-            r3.draw = best_start;
-            restore_pose(C, cand, 1);
-            mutate(C, cand, r3);
-            bfgs(C, cand, A.opt_steps, kVCap, kSlope);
-        }
-        const uint4 b = rng.next();
-        const bool accepted = (cand.e < e_acc) || (u01(b.x) < __expf((e_acc - cand.e) / 1.2f));
-        if (A.step == 0 || accepted) {
-            if (cand.e < best_e) {
-                if (A.opt_steps > 0) bfgs(C, cand, A.opt_steps, kVCapAuth, kSlope);
-                if (cand.e < best_e) {
-                    store_pose(A.L, A.best, lig, chain, cand);
-                    best_e = cand.e;
+            if (A.opt_steps == 0) {
+                float gd;
+                cand.e = energy_grad(C, cand, kVCap, kSlope, gd);
+            } else {
+                bfgs(C, cand, A.opt_steps, kVCap, kSlope);
+            }
+            if (A.policy == kPolicyCollect && ln == 0) A.collect_y[crow] = cand.e;
+            const uint4 b = rng.next();
+            const bool accepted = (cand.e < acc_e) || (u01(b.x) < __expf((acc_e - cand.e) / 1.2f));
+            if (step == 0 || accepted) {
+                if (cand.e < best) {
+                    if (A.opt_steps > 0) bfgs(C, cand, A.opt_steps, kVCapAuth, kSlope);
+                    if (cand.e < best) {
+                        store_pose(A.L, A.P, lig, chain, cand);
+                        best = cand.e;
+                    }
                 }
+                save_pose(C, cand, 1);
             }
-            store_pose(A.L, A.acc, lig, chain, cand);
         }
-        if (ln == 0) A.draw[pid] = rng.draw;
         if (A.stats && ln == 0) {
             const unsigned long long ev = (unsigned long long)C.evals;
+            const unsigned long long nh = (unsigned long long)C.nheavy;
             atomicAdd(&A.stats[0], ev);
-            atomicAdd(&A.stats[1], ev * 32ull * (unsigned long long)C.nheavy);
+            atomicAdd(&A.stats[1], (ev + (unsigned long long)n_feat) * 32ull * nh);
             atomicAdd(&A.stats[2], ev * (unsigned long long)C.npair);
-            if (A.step == 0) atomicAdd(&A.stats[3], 1ull);
+            atomicAdd(&A.stats[3], 1ull);
         }
     }
 }
@@ -292,27 +263,15 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void features_kernel(FeatArg
         default: return hipErrorInvalidValue; \
     }
 
-template <int APL, int K>
+template <int APL>
 static hipError_t launch_lmc_t(const LmcArgs& a, int n_blocks, hipStream_t s) {
     const size_t lds = (size_t)a.lds_floats_per_wave * kWavesPerBlock * sizeof(float);
-    if (K == 0) hipLaunchKernelGGL(lmc_init_kernel<APL>, dim3(n_blocks), dim3(kWave * kWavesPerBlock), lds, s, a);
-    if (K == 1) hipLaunchKernelGGL(lmc_propose_kernel<APL>, dim3(n_blocks), dim3(kWave * kWavesPerBlock), lds, s, a);
-    if (K == 2) hipLaunchKernelGGL(lmc_step_kernel<APL>, dim3(n_blocks), dim3(kWave * kWavesPerBlock), lds, s, a);
+    hipLaunchKernelGGL(lmc_kernel<APL>, dim3(n_blocks), dim3(kWave * kWavesPerBlock), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_lmc_init(LmcArgs a, int apl, int n_blocks, hipStream_t s) {
-#define CALL(N) return launch_lmc_t<N, 0>(a, n_blocks, s)
-    UD2_APL_SWITCH(apl, CALL)
-#undef CALL
-}
-hipError_t launch_lmc_propose(LmcArgs a, int apl, int n_blocks, hipStream_t s) {
-#define CALL(N) return launch_lmc_t<N, 1>(a, n_blocks, s)
-    UD2_APL_SWITCH(apl, CALL)
-#undef CALL
-}
-hipError_t launch_lmc_step(LmcArgs a, int apl, int n_blocks, hipStream_t s) {
-#define CALL(N) return launch_lmc_t<N, 2>(a, n_blocks, s)
+hipError_t launch_lmc(LmcArgs a, int apl, int n_blocks, hipStream_t s) {
+#define CALL(N) return launch_lmc_t<N>(a, n_blocks, s)
     UD2_APL_SWITCH(apl, CALL)
 #undef CALL
 }
@@ -333,26 +292,20 @@ hipError_t launch_features(FeatArgs a, int apl, hipStream_t s) {
 }
 
 template <int APL>
-static int lmc_occ_t(int kernel, int lds_bytes) {
+static int lmc_occ_t(int lds_bytes) {
     int nb = 0;
-    hipError_t e;
-    if (kernel == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lmc_init_kernel<APL>, kWave * kWavesPerBlock, lds_bytes);
-    else if (kernel == 1)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lmc_propose_kernel<APL>, kWave * kWavesPerBlock,
-                                                         lds_bytes);
-    else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lmc_step_kernel<APL>, kWave * kWavesPerBlock, lds_bytes);
-    if (e != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, lmc_kernel<APL>, kWave * kWavesPerBlock, lds_bytes) !=
+        hipSuccess)
+        return 1;
     return nb > 0 ? nb : 1;
 }
 
-int lmc_blocks_per_cu(int kernel, int apl, int lds_bytes) {
+int lmc_blocks_per_cu(int apl, int lds_bytes) {
     switch (apl) {
-        case 1: return lmc_occ_t<1>(kernel, lds_bytes);
-        case 2: return lmc_occ_t<2>(kernel, lds_bytes);
-        case 3: return lmc_occ_t<3>(kernel, lds_bytes);
-        default: return lmc_occ_t<4>(kernel, lds_bytes);
+        case 1: return lmc_occ_t<1>(lds_bytes);
+        case 2: return lmc_occ_t<2>(lds_bytes);
+        case 3: return lmc_occ_t<3>(lds_bytes);
+        default: return lmc_occ_t<4>(lds_bytes);
     }
 }
 

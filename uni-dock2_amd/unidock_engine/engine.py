@@ -51,6 +51,9 @@ class DockParams:
     randomize: bool = True
     mode: int = N.MODE_GRID
     refine_mode: int | None = None  # None: same as mode
+    search: str = "mc"           # "mc": ud2_mc; "plain" / "exact" / "learned": the learned kernel's policies
+    n_cand: int = 16             # candidate mutations per MC step (exact, learned)
+    top_k: int = 4               # candidates the learned policy rescores exactly
 
 
 @dataclass
@@ -65,11 +68,13 @@ class LigandResult:
 class DockEngine:
     """Receptor + box on the device, with grid maps when grid mode is used."""
 
-    def __init__(self, receptor, box, mode: int = N.MODE_GRID, spacing: float = 0.375, probe_types=None):
+    def __init__(self, receptor, box, mode: int = N.MODE_GRID, spacing: float = 0.375, probe_types=None,
+                 proposal=None):
         self.receptor = receptor
         self.box = box
         self.scene = N.Scene(receptor, box)
         self.mode = mode
+        self.proposal = proposal  # N.Mlp for the learned search
         if mode == N.MODE_GRID:
             self.probe_types = tuple(probe_types or DEFAULT_PROBE_TYPES)
             self.scene.build_grids(spacing, self.probe_types)
@@ -105,7 +110,17 @@ class DockEngine:
         if mc_events is not None:
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ev0.record()
-        N.check(lib.ud2_mc(self.scene.h, ligset.h, pb.ref, C.byref(mc), st), "ud2_mc")
+        if prm.search == "mc":
+            N.check(lib.ud2_mc(self.scene.h, ligset.h, pb.ref, C.byref(mc), st), "ud2_mc")
+        else:
+            policy = {"plain": N.POLICY_PLAIN, "exact": N.POLICY_EXACT, "learned": N.POLICY_LEARNED}[prm.search]
+            if policy == N.POLICY_LEARNED and self.proposal is None:
+                raise N.Ud2Error("learned search needs DockEngine.proposal (a trained network)")
+            lp = N.LmcParams()
+            lp.mc = mc
+            lp.policy, lp.n_cand, lp.top_k = policy, prm.n_cand, prm.top_k
+            net = self.proposal.h if policy == N.POLICY_LEARNED else None
+            N.check(lib.ud2_lmc(self.scene.h, ligset.h, pb.ref, C.byref(lp), net, st), "ud2_lmc")
         if mc_events is not None:
             ev1.record()
             mc_events.append((ev0, ev1))
