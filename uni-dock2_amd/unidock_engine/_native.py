@@ -67,6 +67,9 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise Ud2Error(f"libud2.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    # torch ships its own HIP runtime; loading it first makes libud2.so bind to the
+    # same one, so device memory and streams from torch are the library's too
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
     sig = {
