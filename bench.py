@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--learned-nchain", type=int, default=128)
     ap.add_argument("--learned-steps", type=int, default=12)
     ap.add_argument("--n-cand", type=int, default=16)
-    ap.add_argument("--top-k", type=int, default=4)
+    ap.add_argument("--top-k", type=int, default=1)
     ap.add_argument("--weights", default="", help="network parameters (.npz); default: the packaged ones")
     ap.add_argument("--n-pose", type=int, default=1 << 20)
     return ap.parse_args()

@@ -193,6 +193,7 @@ typedef struct {
     int32_t top_k;        /* learned policy: candidates rescored exactly, 1..n_cand */
     uint16_t* collect_x_d;  /* collect: [mc_steps][n_pose][16][80] features (bf16) */
     float* collect_y_d;     /* collect: [mc_steps][n_pose][16] energies after BFGS */
+    float* collect_e0_d;    /* collect, optional: [mc_steps][n_pose][16] energies before BFGS */
 } ud2_lmc_params;
 
 /* Network on the current device from params_h[UD2_MLP_NPARAM + UD2_MLP_NNORM]. */

@@ -1034,6 +1034,7 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
     A.n_pose = npose;
     A.collect_x = prm->collect_x_d;
     A.collect_y = prm->collect_y_d;
+    A.collect_e0 = prm->collect_e0_d;
     A.stats = reinterpret_cast<unsigned long long*>(mc.stats_d);
     std::vector<int32_t> order(ls->n_lig);
     std::iota(order.begin(), order.end(), 0);

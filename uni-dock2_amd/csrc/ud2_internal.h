@@ -199,6 +199,7 @@ struct LmcArgs {
     int64_t n_pose;
     uint16_t* collect_x;     // collect: [mc_steps][n_pose][16][80] features
     float* collect_y;        // collect: [mc_steps][n_pose][16] energies after BFGS
+    float* collect_e0;       // collect (optional): [mc_steps][n_pose][16] energies before BFGS
     unsigned long long* stats;
 };
 

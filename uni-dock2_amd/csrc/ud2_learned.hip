@@ -196,6 +196,11 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
                         WPose<APL> t;
                         restore_pose(C, t, 1);
                         mutate(C, t, r2);
+                        if (A.collect_e0) {
+                            Forces<APL> F;
+                            const float e0 = energy_forces(C, t, kVCap, kSlope, F);
+                            if (ln == 0) A.collect_e0[crow + c] = e0;
+                        }
                         bfgs(C, t, A.opt_steps, kVCap, kSlope);
                         if (ln == 0) A.collect_y[crow + c] = t.e;
                     }
@@ -203,6 +208,11 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
                     r2.draw = uint32_t(rdl_i(int(my_start), 0));
                     restore_pose(C, cand, 1);
                     mutate(C, cand, r2);
+                    if (A.collect_e0) {
+                        Forces<APL> F;
+                        const float e0 = energy_forces(C, cand, kVCap, kSlope, F);
+                        if (ln == 0) A.collect_e0[crow] = e0;
+                    }
                 }
             }
             if (A.opt_steps == 0) {

@@ -56,10 +56,11 @@ class Dataset:
     x: object        # int16 device tensor [groups * 16, 80], bf16 bits
     y: object        # float32 device tensor [groups * 16], energies after BFGS
     n_groups: int
+    e0: object = None  # float32 device tensor [groups * 16], energies before BFGS (optional)
 
 
 def collect(engine, ligands, nchain: int = 64, steps: int = 8, opt_steps: int = -1, seed: int = 1,
-            lig_index_base: int = HELD_OUT_SEED, stream=None) -> Dataset:
+            lig_index_base: int = HELD_OUT_SEED, with_e0: bool = False, stream=None) -> Dataset:
     """Runs the collect policy of the learned MC kernel over the ligands."""
     import torch
     from .engine import heuristic_opt_steps
@@ -70,6 +71,7 @@ def collect(engine, ligands, nchain: int = 64, steps: int = 8, opt_steps: int = 
     npose = ls.n_lig * nchain
     x = torch.zeros((steps * npose * N.MLP_GROUP, N.MLP_FEATURES), dtype=torch.int16, device="cuda")
     y = torch.zeros(steps * npose * N.MLP_GROUP, dtype=torch.float32, device="cuda")
+    e0 = torch.zeros_like(y) if with_e0 else None
     if opt_steps < 0:
         opt_steps = heuristic_opt_steps(max(l.natom for l in ligands))
     prm = N.LmcParams()
@@ -77,15 +79,25 @@ def collect(engine, ligands, nchain: int = 64, steps: int = 8, opt_steps: int = 
                         mode=N.MODE_GRID, lig_index_base=lig_index_base, stats_d=None)
     prm.policy, prm.n_cand, prm.top_k = N.POLICY_COLLECT, N.MLP_GROUP, 1
     prm.collect_x_d, prm.collect_y_d = x.data_ptr(), y.data_ptr()
+    prm.collect_e0_d = e0.data_ptr() if with_e0 else None
     N.check(N.lib().ud2_lmc(engine.scene.h, ls.h, pb.ref, C.byref(prm), None, N.stream_ptr(stream)), "ud2_lmc")
     torch.cuda.synchronize()
-    return Dataset(x, y, steps * npose)
+    return Dataset(x, y, steps * npose, e0)
 
 
-def fit_normalization(params: np.ndarray, data: Dataset, sample_groups: int = 65536, seed: int = 0) -> None:
-    """Sets mu / inv_sigma from a sample of the training inputs (in place)."""
+def finite_groups(data: Dataset) -> np.ndarray:
+    """Groups whose 16 feature rows are all finite and that have a finite label."""
     import torch
-    g = torch.as_tensor(np.random.default_rng(seed).choice(data.n_groups, min(sample_groups, data.n_groups),
+    xf = bf16_to_float(data.x).view(data.n_groups, N.MLP_GROUP * N.MLP_FEATURES)
+    ok = torch.isfinite(xf).all(dim=1) & torch.isfinite(data.y.view(data.n_groups, N.MLP_GROUP)).any(dim=1)
+    return torch.nonzero(ok).view(-1).cpu().numpy().astype(np.int32)
+
+
+def fit_normalization(params: np.ndarray, data: Dataset, groups: np.ndarray, sample_groups: int = 65536,
+                      seed: int = 0) -> None:
+    """Sets mu / inv_sigma from a sample of the training groups' inputs (in place)."""
+    import torch
+    g = torch.as_tensor(np.random.default_rng(seed).choice(groups, min(sample_groups, len(groups)),
                                                            replace=False), device="cuda")
     rows = (g[:, None] * N.MLP_GROUP + torch.arange(N.MLP_GROUP, device="cuda")[None, :]).reshape(-1)
     xs = bf16_to_float(data.x[rows])
@@ -97,9 +109,9 @@ def fit_normalization(params: np.ndarray, data: Dataset, sample_groups: int = 65
     params[N.MLP_NPARAM + F:] = inv.cpu().numpy()
 
 
-def split_groups(n_groups: int, val_frac: float = 0.05, seed: int = 0):
-    perm = np.random.default_rng(seed).permutation(n_groups).astype(np.int32)
-    nv = max(1, int(n_groups * val_frac))
+def split_groups(groups: np.ndarray, val_frac: float = 0.05, seed: int = 0):
+    perm = np.random.default_rng(seed).permutation(groups).astype(np.int32)
+    nv = max(1, int(len(perm) * val_frac))
     return perm[nv:], perm[:nv]
 
 
@@ -179,12 +191,13 @@ def load(path: str = WEIGHTS) -> np.ndarray:
     return p
 
 
-def build_training_set(engine, n_lig: int = 512, nchain: int = 128, steps: int = 10, seed: int = 1, log=print):
+def build_training_set(engine, n_lig: int = 512, nchain: int = 128, steps: int = 10, seed: int = 1, log=print,
+                       with_e0: bool = False):
     """Collects the held-out training set (c1-style ligands, seeds 9000 + i)."""
     from . import synthetic as S
     ligs = [s.ligand for s in S.config1_ligands(n_lig, base_seed=HELD_OUT_SEED)]
     t0 = time.time()
-    data = collect(engine, ligs, nchain=nchain, steps=steps, seed=seed)
+    data = collect(engine, ligs, nchain=nchain, steps=steps, seed=seed, with_e0=with_e0)
     log(f"collected {data.n_groups} groups ({data.n_groups * N.MLP_GROUP} candidates) in {time.time() - t0:.1f} s")
     return data
 
@@ -200,14 +213,27 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--dump-groups", type=int, default=0, help="also save this many random groups (x, y, e0)")
+    ap.add_argument("--dump", default="")
     a = ap.parse_args(argv)
     from . import synthetic as S
     from .engine import DockEngine
     eng = DockEngine(S.make_receptor(), S.docking_box(), mode=N.MODE_GRID)
-    data = build_training_set(eng, a.n_lig, a.nchain, a.steps, seed=a.seed + 1, log=lambda s: print(s, flush=True))
-    tr, va = split_groups(data.n_groups, seed=a.seed)
+    data = build_training_set(eng, a.n_lig, a.nchain, a.steps, seed=a.seed + 1, log=lambda s: print(s, flush=True),
+                              with_e0=a.dump_groups > 0)
+    if a.dump_groups > 0:
+        import torch
+        g = np.sort(np.random.default_rng(a.seed + 7).choice(data.n_groups, a.dump_groups, replace=False))
+        rows = (g[:, None] * N.MLP_GROUP + np.arange(N.MLP_GROUP)[None, :]).reshape(-1)
+        r = torch.as_tensor(rows, device="cuda")
+        np.savez_compressed(a.dump, x=data.x[r].cpu().numpy(), y=data.y[r].cpu().numpy(),
+                            e0=data.e0[r].cpu().numpy(), groups=g)
+        print("dumped", a.dump, flush=True)
+    good = finite_groups(data)
+    print(f"{len(good)} of {data.n_groups} groups have finite features and labels", flush=True)
+    tr, va = split_groups(good, seed=a.seed)
     params = init_params(a.seed)
-    fit_normalization(params, data, seed=a.seed)
+    fit_normalization(params, data, tr, seed=a.seed)
     mlp = N.Mlp(params)
     before = ranking_quality(scores(mlp, data, va), data.y.view(-1, N.MLP_GROUP)[va].cpu().numpy())
     print("untrained:", before, flush=True)
