@@ -12,9 +12,14 @@ scoring.
 
   plain    128 chains x 40 MC steps, one random mutation + BFGS of
            (25 + natom_max) / 3 iterations per step (the reference's search);
-  learned  the learned proposal: per step 16 mutations scored by the network
-           on MFMA, the top 4 rescored with exact Vina, the best BFGS-refined;
-           --learned-nchain chains x --learned-steps MC steps.
+  learned  the learned engine: 32 chains x 40 MC steps; each chain starts from
+           the best of 4 randomised starts by the network, and each step draws
+           16 mutations, scores them with the network on MFMA and samples one
+           with p ~ exp(-score / 1 kcal/mol) for BFGS and the Metropolis test.
+
+The plain engine is also timed at the learned engine's budget (32 x 40,
+``config.plain_same_budget``): on this workload that control shows how much
+of the speedup comes from the smaller budget rather than from the network.
 
 ``value`` is the learned engine's ligands/s; the plain engine is timed on the
 same batch in the same run (``config.plain``) with the median top-pose
@@ -53,15 +58,21 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=2, help="ligands timed on the CPU oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["batch", "score"], default="batch",
-                    help="batch: docking throughput (headline); score: E+gradient microbenchmark")
+    ap.add_argument("--workload", choices=["batch", "score", "copy"], default="batch",
+                    help="batch: docking throughput (headline); score: E+gradient microbenchmark; copy: a 256 MiB "
+                         "device copy that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--search", choices=["learned", "plain"], default="learned",
                     help="learned: headline (learned vs plain); plain: the plain engine only")
-    ap.add_argument("--learned-nchain", type=int, default=128)
-    ap.add_argument("--learned-steps", type=int, default=12)
+    ap.add_argument("--learned-nchain", type=int, default=32)
+    ap.add_argument("--learned-steps", type=int, default=40)
     ap.add_argument("--n-cand", type=int, default=16)
     ap.add_argument("--top-k", type=int, default=1)
     ap.add_argument("--weights", default="", help="network parameters (.npz); default: the packaged ones")
+    ap.add_argument("--sel-temp", type=float, default=1.0,
+                    help="learned: sample one candidate with p ~ exp(-score / T) (0: top-k exact rescoring)")
+    ap.add_argument("--n-start", type=int, default=4, help="learned: randomised starts per chain, best by the network")
+    ap.add_argument("--no-control", action="store_true",
+                    help="learned: skip the plain engine at the learned engine's chain / step budget")
     ap.add_argument("--n-pose", type=int, default=1 << 20)
     return ap.parse_args()
 
@@ -170,6 +181,33 @@ def score_microbench(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe():
+    """A plain 256 MiB device-to-device copy, three times (counter calibration)."""
+    import torch
+    n = 256 << 20
+    a = torch.empty(n // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_per_copy": n, "copies": 3}), flush=True)
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lmc_mc_kernels_c1_batch_traffic.md")
+
+
+def profiled_traffic(key: str):
+    """Bytes per launch measured with the FETCH_SIZE / WRITE_SIZE counters for the default
+    workload (profiles/), or None."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith(key + "="):
+                return int(float(line.split("=", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def timed_dock(eng, ls, prm, pb, steps, warmup, tag):
     """Warmup + timed docking passes; returns (wall seconds, results, MC launch events, kernel counters)."""
     import torch
@@ -196,6 +234,8 @@ def main():
         sys.exit(2)
     if args.workload == "score":
         return score_microbench(args)
+    if args.workload == "copy":
+        return copy_probe()
     import numpy as np
     from unidock_engine import _native as N
     from unidock_engine import proposal as LP
@@ -222,13 +262,23 @@ def main():
     if learned:
         eng.proposal = N.Mlp(LP.load(args.weights or LP.WEIGHTS))
         lprm = DockParams(nchain=args.learned_nchain, mc_steps=args.learned_steps, search="learned",
-                          n_cand=args.n_cand, top_k=args.top_k, **base)
+                          n_cand=args.n_cand, top_k=args.top_k, sel_temp=args.sel_temp, n_start=args.n_start, **base)
     # plain engine: one timed pass when it is only the comparison, all passes otherwise
     pb_plain = N.PoseBuffers(ls, args.nchain)
     p_steps, p_warm = (1, 1) if learned else (args.steps, args.warmup)
     p_wall, p_res, p_ev, p_st = timed_dock(eng, ls, plain_prm, pb_plain, p_steps, p_warm, "plain")
     p_value = len(ligands) * p_steps / p_wall
     p_top = np.asarray([float(r.energies[0][0]) if r.energies else np.nan for r in p_res])
+    control = None
+    if learned and not args.no_control:
+        # the plain engine at the learned engine's chain / step budget: separates what the
+        # network contributes from what the smaller budget does
+        cprm = DockParams(nchain=args.learned_nchain, mc_steps=args.learned_steps, search="mc", **base)
+        pbc = N.PoseBuffers(ls, args.learned_nchain)
+        c_wall, c_res, _, _ = timed_dock(eng, ls, cprm, pbc, 1, 1, "plain-same-budget")
+        c_top = np.asarray([float(r.energies[0][0]) if r.energies else np.nan for r in c_res])
+        control = {"value": round(len(ligands) / c_wall, 4), "unit": "ligands/s", "nchain": args.learned_nchain,
+                   "mc_steps": args.learned_steps, "median_top_affinity": round(float(np.nanmedian(c_top)), 4)}
     if learned:
         pb = N.PoseBuffers(ls, args.learned_nchain)
         wall, res, ev, st = timed_dock(eng, ls, lprm, pb, args.steps, args.warmup, "learned")
@@ -248,8 +298,11 @@ def main():
     cfg = {
         "workload": f"{len(ligands)}-ligand uneven batch (10-60 heavy atoms, 0-15 rotatable bonds), MC search + "
                     f"cluster + refine + score, {args.mode} inter term",
-        "method": (f"learned proposal: {nchain} chains x {msteps} MC steps, {args.n_cand} candidates per step "
-                   f"scored by the network, top {args.top_k} rescored exactly, best BFGS {opt_steps}"
+        "method": (f"learned proposal: {nchain} chains x {msteps} MC steps, each chain started from the best of "
+                   f"{args.n_start} randomised starts by the network; per step {args.n_cand} candidate mutations "
+                   f"scored by the network, " + (f"one sampled with p ~ exp(-score / {args.sel_temp} kcal/mol)"
+                                                 if args.sel_temp > 0 else f"top {args.top_k} rescored exactly") +
+                   f", BFGS {opt_steps}"
                    if learned else f"plain MC: {nchain} chains x {msteps} MC steps x BFGS {opt_steps}"),
         "n_ligands": len(ligands),
         "nchain": nchain,
@@ -271,6 +324,8 @@ def main():
         cfg["median_top_affinity_diff"] = round(float(np.nanmedian(top) - np.nanmedian(p_top)), 4)
         cfg["median_per_ligand_top_diff"] = round(float(np.median(top[ok] - p_top[ok])), 4)
         cfg["frac_ligands_within_0.1"] = round(float(np.mean(top[ok] - p_top[ok] <= 0.1)), 4)
+        if control is not None:
+            cfg["plain_same_budget"] = control
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -296,6 +351,13 @@ def main():
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
     }
+    default_run = (args.n_lig, args.mode, args.nchain, args.mc_steps, args.learned_nchain, args.learned_steps,
+                   args.n_cand, args.sel_temp, args.n_start) == (1024, "grid", 128, 40, 32, 40, 16, 1.0, 4)
+    if default_run:
+        t = profiled_traffic("lmc_traffic_bytes_per_launch" if learned else "mc_traffic_bytes_per_launch")
+        if t:
+            out["roofline"]["traffic"] = t
+            out["roofline"]["traffic_source"] = "profiles/lmc_mc_kernels_c1_batch_traffic.md (FETCH_SIZE + WRITE_SIZE)"
     if not args.no_cpu and args.cpu_sample > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(ligands, receptor, box, args.nchain, args.mc_steps, opt_steps,

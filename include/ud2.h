@@ -191,6 +191,9 @@ typedef struct {
     int32_t policy;       /* UD2_POLICY_* */
     int32_t n_cand;       /* mutations per step, 1..16 (ignored by the plain policy) */
     int32_t top_k;        /* learned policy: candidates rescored exactly, 1..n_cand */
+    float sel_temp;       /* learned policy: > 0 instead draws one candidate with p ~ exp(-score / sel_temp) */
+    int32_t n_start;      /* > 1: randomised starts drawn per chain (orientations spread over n_start * nchain);
+                             the chain starts from the best by the network (learned) or the exact energy */
     uint16_t* collect_x_d;  /* collect: [mc_steps][n_pose][16][80] features (bf16) */
     float* collect_y_d;     /* collect: [mc_steps][n_pose][16] energies after BFGS */
     float* collect_e0_d;    /* collect, optional: [mc_steps][n_pose][16] energies before BFGS */

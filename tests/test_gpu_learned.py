@@ -316,3 +316,33 @@ def test_lmc_policies_keep_exact_poses(gpu, scene, lig_c0, lig_c2, policy):
         feats = bf16_to_f32(collect[0])
         assert np.isfinite(lab).all() and np.isfinite(feats).all()
         assert np.all(feats.reshape(-1, NF)[:, 0:64:8] >= 0)  # atom counts per class
+
+
+def test_trained_network_beats_untrained_on_held_out_groups(gpu, scene):
+    """What the trained network gains over the method without it, as a predictor: on MC
+    states it never saw (ligand seeds 9500+, outside its training set), the candidate it
+    ranks first ends closer to the group's best energy after BFGS than the first of an
+    untrained network with the same input normalisation or of a random order (the plain
+    MC's choice). Both gains are asserted with a margin of ~4 standard errors."""
+    import sys
+    from unidock_engine import proposal as LP
+    from unidock_engine import synthetic as S
+    from unidock_engine.engine import DockEngine
+    print("held-out collection", flush=True, file=sys.stderr)
+    ligs = [s.ligand for s in S.config1_ligands(16, base_seed=9500)]
+    eng = DockEngine.__new__(DockEngine)
+    eng.scene, eng.mode, eng.probe_types = scene, gpu.MODE_GRID, (2, 3, 4, 5, 6, 10, 11, 15)
+    data = LP.collect(eng, ligs, nchain=32, steps=4, seed=3, lig_index_base=9500)
+    groups = LP.finite_groups(data)
+    assert len(groups) > 0.95 * data.n_groups
+    lab = data.y.view(-1, G)[groups].cpu().numpy()
+    wts = LP.load()
+    trained = gpu.Mlp(wts)
+    u = LP.init_params(0)
+    u[NP:] = wts[NP:]
+    untrained = gpu.Mlp(u)
+    qt = LP.ranking_quality(LP.scores(trained, data, groups), lab)
+    qu = LP.ranking_quality(LP.scores(untrained, data, groups), lab)
+    print("trained", qt, "untrained", qu, flush=True, file=sys.stderr)
+    assert qt["regret_top1"] < qt["random_regret_top1"] - 0.2
+    assert qt["regret_top1"] < qu["regret_top1"] - 0.2

@@ -1003,8 +1003,11 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
     if (policy < UD2_POLICY_PLAIN || policy > UD2_POLICY_COLLECT) return fail(UD2_ERR_ARG, "unknown policy");
     const int n_cand = policy == UD2_POLICY_PLAIN ? 1 : prm->n_cand;
     if (n_cand < 1 || n_cand > kMaxCand) return fail(UD2_ERR_ARG, "n_cand must be in 1..16");
-    if (policy == UD2_POLICY_LEARNED && (prm->top_k < 1 || prm->top_k > n_cand))
+    if (policy == UD2_POLICY_LEARNED && !(prm->sel_temp > 0.0f) && (prm->top_k < 1 || prm->top_k > n_cand))
         return fail(UD2_ERR_ARG, "top_k must be in 1..n_cand");
+    if (!(prm->sel_temp >= 0.0f) || !std::isfinite(prm->sel_temp)) return fail(UD2_ERR_ARG, "sel_temp must be >= 0");
+    if (prm->n_start > kMaxCand) return fail(UD2_ERR_ARG, "n_start must be <= 16");
+    if (prm->n_start > 1 && policy == UD2_POLICY_COLLECT) return fail(UD2_ERR_ARG, "collect policy with n_start > 1");
     if (policy == UD2_POLICY_LEARNED && !mlp) return fail(UD2_ERR_ARG, "learned policy without a network");
     const bool feats = policy == UD2_POLICY_LEARNED || policy == UD2_POLICY_COLLECT;
     if (feats) {
@@ -1030,6 +1033,8 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
     A.policy = policy;
     A.n_cand = n_cand;
     A.top_k = prm->top_k;
+    A.sel_temp = prm->sel_temp;
+    A.n_start = prm->n_start < 1 ? 1 : prm->n_start;
     if (mlp) A.W = mlp->view();
     A.n_pose = npose;
     A.collect_x = prm->collect_x_d;

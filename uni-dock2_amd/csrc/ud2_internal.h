@@ -195,6 +195,8 @@ struct LmcArgs {
     int32_t hdim_ld;
     int32_t lscr, lfeat, la1;  // feature scratch, candidate rows, hidden layer (floats from the slice start)
     int32_t policy, n_cand, top_k;
+    float sel_temp;          // learned: > 0 draws one candidate with p ~ exp(-score / sel_temp)
+    int32_t n_start;         // > 1: randomised starts per chain, the best kept (network or exact energy)
     MlpDev W;
     int64_t n_pose;
     uint16_t* collect_x;     // collect: [mc_steps][n_pose][16][80] features

@@ -117,14 +117,55 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
         Ctx<APL> C;
         load_ctx(C, A.L, A.S, lig, A.mode, base, A.hdim_ld);
         Rng rng{A.seed, uint32_t(lig + A.lig_index_base), uint32_t(chain), 0u};
+        C.evals = 0;
+        int n_feat = 0;
         WPose<APL> acc;
         initial_pose(C, A.L, lig, acc);
-        if (A.randomize) randomize(C, acc, rng, chain, nchain);
+        if (A.randomize && A.n_start > 1) {
+            // start pre-screen: n_start randomised starts (super-Fibonacci orientations spread
+            // over n_start * nchain), the best by the network (learned) or by the exact energy
+            save_pose(C, acc, 1);
+            const int ns = A.n_start;
+            uint32_t my_start = 0;
+            float e_exact = INFINITY;
+            for (int k = 0; k < ns; k++) {
+                if (ln == k) my_start = rng.draw;
+                WPose<APL> t;
+                restore_pose(C, t, 1);
+                randomize(C, t, rng, chain * ns + k, nchain * ns);
+                if (A.policy == kPolicyLearned) {
+                    Move mv;
+                    mv.kind = 0; mv.mag = 1.0f; mv.moved = 1.0f;
+                    pose_features(C, t, mv, 999.0f, base + A.lscr, xl + k * kFeat);
+                } else {
+                    Forces<APL> F;
+                    const float e = energy_forces(C, t, kVCap, kSlope, F);
+                    if (ln == k) e_exact = e;
+                }
+            }
+            n_feat += A.policy == kPolicyLearned ? ns : 0;
+            float p = e_exact;
+            if (A.policy == kPolicyLearned) {
+                wave_lds_fence();
+                p = mlp_group_scores_lds(A.W, xl, a1l);
+            }
+            p = (ln < ns && p == p) ? p : INFINITY;
+            int pick = 0;
+            float pbest = INFINITY;
+            for (int k = 0; k < ns; k++) {
+                const float pk = rdl(p, k);
+                if (pk < pbest) { pbest = pk; pick = k; }
+            }
+            Rng r2 = rng;
+            r2.draw = uint32_t(rdl_i(int(my_start), pick));
+            restore_pose(C, acc, 1);
+            randomize(C, acc, r2, chain * ns + pick, nchain * ns);
+        } else if (A.randomize) {
+            randomize(C, acc, rng, chain, nchain);
+        }
         store_pose(A.L, A.P, lig, chain, acc);
         save_pose(C, acc, 1);  // the accepted pose lives in LDS between steps
         float best = 1e9f;
-        C.evals = 0;
-        int n_feat = 0;
         for (int step = 0; step < A.mc_steps; step++) {
             WPose<APL> cand;
             restore_pose(C, cand, 1);
@@ -165,13 +206,30 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
                     float p = mlp_group_scores_lds(A.W, xl, a1l);  // lane 16 q + r: score of candidate r
                     p = (ln < A.n_cand && p == p) ? p : INFINITY;
                     int rank = 0;
+                    float pmin = INFINITY;
                     for (int j = 0; j < A.n_cand; j++) {
                         const float pj = rdl(p, j);
                         rank += (pj < p || (pj == p && j < ln)) ? 1 : 0;
+                        pmin = fminf(pmin, pj);
                     }
                     float be = INFINITY;
                     uint32_t bs = 0;
-                    for (int r = 0; r < A.top_k; r++) {
+                    if (A.sel_temp > 0.0f) {
+                        // one candidate drawn with p ~ exp(-(score - min) / T); no exact rescoring
+                        const float w = (ln < A.n_cand && pmin < INFINITY) ? __expf(-(p - pmin) / A.sel_temp) : 0.0f;
+                        float wsum = 0.0f;
+                        for (int j = 0; j < A.n_cand; j++) wsum += rdl(w, j);
+                        const float target = u01(rng.next().x) * wsum;
+                        int pick = 0;
+                        float cum = 0.0f;
+                        for (int j = 0; j < A.n_cand; j++) {
+                            cum += rdl(w, j);
+                            if (cum < target) pick = j + 1;
+                        }
+                        pick = pick < A.n_cand ? pick : A.n_cand - 1;
+                        bs = uint32_t(rdl_i(int(my_start), pick));
+                    }
+                    for (int r = 0; A.sel_temp <= 0.0f && r < A.top_k; r++) {
                         const unsigned long long m = __ballot(ln < A.n_cand && rank == r);
                         const int c = __ffsll((long long)m) - 1;
                         const uint32_t start = uint32_t(rdl_i(int(my_start), c));

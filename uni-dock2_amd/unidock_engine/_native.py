@@ -54,7 +54,7 @@ class Adam(C.Structure):
 
 class LmcParams(C.Structure):
     _fields_ = [("mc", McParams), ("policy", C.c_int32), ("n_cand", C.c_int32), ("top_k", C.c_int32),
-                ("collect_x_d", C.c_void_p), ("collect_y_d", C.c_void_p), ("collect_e0_d", C.c_void_p)]
+                ("sel_temp", C.c_float), ("n_start", C.c_int32), ("collect_x_d", C.c_void_p), ("collect_y_d", C.c_void_p), ("collect_e0_d", C.c_void_p)]
 
 
 _lib = None

@@ -54,6 +54,8 @@ class DockParams:
     search: str = "mc"           # "mc": ud2_mc; "plain" / "exact" / "learned": the learned kernel's policies
     n_cand: int = 16             # candidate mutations per MC step (exact, learned)
     top_k: int = 4               # candidates the learned policy rescores exactly
+    sel_temp: float = 0.0        # learned: > 0 draws one candidate with p ~ exp(-score / sel_temp) instead
+    n_start: int = 1             # > 1: randomised starts per chain, the best kept
 
 
 @dataclass
@@ -118,7 +120,8 @@ class DockEngine:
                 raise N.Ud2Error("learned search needs DockEngine.proposal (a trained network)")
             lp = N.LmcParams()
             lp.mc = mc
-            lp.policy, lp.n_cand, lp.top_k = policy, prm.n_cand, prm.top_k
+            lp.policy, lp.n_cand, lp.top_k, lp.sel_temp = policy, prm.n_cand, prm.top_k, prm.sel_temp
+            lp.n_start = prm.n_start
             net = self.proposal.h if policy == N.POLICY_LEARNED else None
             N.check(lib.ud2_lmc(self.scene.h, ligset.h, pb.ref, C.byref(lp), net, st), "ud2_lmc")
         if mc_events is not None:
