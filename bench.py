@@ -196,16 +196,23 @@ def copy_probe():
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lmc_mc_kernels_c1_batch_traffic.md")
 
 
+DEFAULT_TRAFFIC_CONFIG = "1024 ligands grid; learned 32x40 starts 4 cand 16 T 1.0; plain 128x40"
+
+
 def profiled_traffic(key: str):
     """Bytes per launch measured with the FETCH_SIZE / WRITE_SIZE counters for the default
-    workload (profiles/), or None."""
+    workload (profiles/), or None when that summary is for another configuration."""
+    vals = {}
     try:
         for line in open(TRAFFIC_SUMMARY):
-            if line.startswith(key + "="):
-                return int(float(line.split("=", 1)[1]))
+            if "=" in line and not line.startswith(("|", "`", " ")):
+                k, v = line.strip().split("=", 1)
+                vals[k] = v
     except OSError:
-        pass
-    return None
+        return None
+    if vals.get("config") != DEFAULT_TRAFFIC_CONFIG or key not in vals:
+        return None
+    return int(float(vals[key]))
 
 
 def timed_dock(eng, ls, prm, pb, steps, warmup, tag):
@@ -354,7 +361,7 @@ def main():
     default_run = (args.n_lig, args.mode, args.nchain, args.mc_steps, args.learned_nchain, args.learned_steps,
                    args.n_cand, args.sel_temp, args.n_start) == (1024, "grid", 128, 40, 32, 40, 16, 1.0, 4)
     if default_run:
-        t = profiled_traffic("lmc_traffic_bytes_per_launch" if learned else "mc_traffic_bytes_per_launch")
+        t = profiled_traffic("lmc_traffic_bytes_per_launch") if learned else None
         if t:
             out["roofline"]["traffic"] = t
             out["roofline"]["traffic_source"] = "profiles/lmc_mc_kernels_c1_batch_traffic.md (FETCH_SIZE + WRITE_SIZE)"

@@ -121,56 +121,42 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
         int n_feat = 0;
         WPose<APL> acc;
         initial_pose(C, A.L, lig, acc);
-        if (A.randomize && A.n_start > 1) {
-            // start pre-screen: n_start randomised starts (super-Fibonacci orientations spread
-            // over n_start * nchain), the best by the network (learned) or by the exact energy
-            save_pose(C, acc, 1);
-            const int ns = A.n_start;
-            uint32_t my_start = 0;
-            float e_exact = INFINITY;
-            for (int k = 0; k < ns; k++) {
-                if (ln == k) my_start = rng.draw;
-                WPose<APL> t;
-                restore_pose(C, t, 1);
-                randomize(C, t, rng, chain * ns + k, nchain * ns);
-                if (A.policy == kPolicyLearned) {
-                    Move mv;
-                    mv.kind = 0; mv.mag = 1.0f; mv.moved = 1.0f;
-                    pose_features(C, t, mv, 999.0f, base + A.lscr, xl + k * kFeat);
-                } else {
-                    Forces<APL> F;
-                    const float e = energy_forces(C, t, kVCap, kSlope, F);
-                    if (ln == k) e_exact = e;
-                }
-            }
-            n_feat += A.policy == kPolicyLearned ? ns : 0;
-            float p = e_exact;
-            if (A.policy == kPolicyLearned) {
-                wave_lds_fence();
-                p = mlp_group_scores_lds(A.W, xl, a1l);
-            }
-            p = (ln < ns && p == p) ? p : INFINITY;
-            int pick = 0;
-            float pbest = INFINITY;
-            for (int k = 0; k < ns; k++) {
-                const float pk = rdl(p, k);
-                if (pk < pbest) { pbest = pk; pick = k; }
-            }
-            Rng r2 = rng;
-            r2.draw = uint32_t(rdl_i(int(my_start), pick));
-            restore_pose(C, acc, 1);
-            randomize(C, acc, r2, chain * ns + pick, nchain * ns);
-        } else if (A.randomize) {
-            randomize(C, acc, rng, chain, nchain);
-        }
+        // step -1 is the start pre-screen: n_start randomised starts (super-Fibonacci
+        // orientations spread over n_start * nchain) go through the same candidate code
+        // as the MC steps' mutations, and the chain starts from the best one (network,
+        // or exact energy for the plain / exact policies)
+        const int ns = A.n_start;
+        const bool prescreen = A.randomize && ns > 1;
+        if (A.randomize && !prescreen) randomize(C, acc, rng, chain, nchain);
         store_pose(A.L, A.P, lig, chain, acc);
         save_pose(C, acc, 1);  // the accepted pose lives in LDS between steps
         float best = 1e9f;
-        for (int step = 0; step < A.mc_steps; step++) {
+        for (int step = prescreen ? -1 : 0; step < A.mc_steps; step++) {
+            const bool pre = step < 0;
             WPose<APL> cand;
             restore_pose(C, cand, 1);
             const float acc_e = cand.e;
-            const size_t crow = ((size_t)step * A.n_pose + pid) * kGroup;
+            const size_t crow = ((size_t)(pre ? 0 : step) * A.n_pose + pid) * kGroup;
+            if (pre && A.policy != kPolicyLearned) {
+                float be = INFINITY;
+                uint32_t bs = rng.draw;
+                int bk = 0;
+                for (int k = 0; k < ns; k++) {
+                    const uint32_t start = rng.draw;
+                    WPose<APL> t;
+                    restore_pose(C, t, 1);
+                    randomize(C, t, rng, chain * ns + k, nchain * ns);
+                    Forces<APL> F;
+                    const float e = energy_forces(C, t, kVCap, kSlope, F);
+                    if (e < be || k == 0) { be = e; bs = start; bk = k; }
+                }
+                Rng r3 = rng;
+                r3.draw = bs;
+                randomize(C, cand, r3, chain * ns + bk, nchain * ns);
+                store_pose(A.L, A.P, lig, chain, cand);
+                save_pose(C, cand, 1);
+                continue;
+            }
             if (A.policy == kPolicyPlain) {
                 mutate(C, cand, rng);
             } else if (A.policy == kPolicyExact) {
@@ -191,46 +177,64 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
                 mutate(C, cand, r3);
             } else {
                 // candidate c starts at Philox block my_start of lane c
+                const int nc = pre ? ns : A.n_cand;
                 uint32_t my_start = 0;
-                for (int c = 0; c < A.n_cand; c++) {
+                for (int c = 0; c < nc; c++) {
                     if (ln == c) my_start = rng.draw;
                     WPose<APL> t;
                     restore_pose(C, t, 1);
-                    const Move mv = mutate(C, t, rng);
+                    Move mv;
+                    if (pre) {
+                        randomize(C, t, rng, chain * ns + c, nchain * ns);
+                        mv.kind = 0; mv.mag = 1.0f; mv.moved = 1.0f;
+                    } else {
+                        mv = mutate(C, t, rng);
+                    }
                     uint16_t* dst = A.policy == kPolicyCollect ? A.collect_x + (crow + c) * kFeat : xl + c * kFeat;
-                    pose_features(C, t, mv, acc_e, base + A.lscr, dst);
+                    pose_features(C, t, mv, pre ? 999.0f : acc_e, base + A.lscr, dst);
                 }
-                n_feat += A.n_cand;
+                n_feat += nc;
                 if (A.policy == kPolicyLearned) {
                     wave_lds_fence();
                     float p = mlp_group_scores_lds(A.W, xl, a1l);  // lane 16 q + r: score of candidate r
-                    p = (ln < A.n_cand && p == p) ? p : INFINITY;
+                    p = (ln < nc && p == p) ? p : INFINITY;
                     int rank = 0;
                     float pmin = INFINITY;
-                    for (int j = 0; j < A.n_cand; j++) {
+                    for (int j = 0; j < nc; j++) {
                         const float pj = rdl(p, j);
                         rank += (pj < p || (pj == p && j < ln)) ? 1 : 0;
                         pmin = fminf(pmin, pj);
                     }
                     float be = INFINITY;
                     uint32_t bs = 0;
+                    int pick = 0;
+                    if (pre) {
+                        pick = __ffsll((long long)__ballot(ln < nc && rank == 0)) - 1;
+                        bs = uint32_t(rdl_i(int(my_start), pick));
+                        Rng r3 = rng;
+                        r3.draw = bs;
+                        restore_pose(C, cand, 1);
+                        randomize(C, cand, r3, chain * ns + pick, nchain * ns);
+                        store_pose(A.L, A.P, lig, chain, cand);
+                        save_pose(C, cand, 1);
+                        continue;
+                    }
                     if (A.sel_temp > 0.0f) {
                         // one candidate drawn with p ~ exp(-(score - min) / T); no exact rescoring
-                        const float w = (ln < A.n_cand && pmin < INFINITY) ? __expf(-(p - pmin) / A.sel_temp) : 0.0f;
+                        const float w = (ln < nc && pmin < INFINITY) ? __expf(-(p - pmin) / A.sel_temp) : 0.0f;
                         float wsum = 0.0f;
-                        for (int j = 0; j < A.n_cand; j++) wsum += rdl(w, j);
+                        for (int j = 0; j < nc; j++) wsum += rdl(w, j);
                         const float target = u01(rng.next().x) * wsum;
-                        int pick = 0;
                         float cum = 0.0f;
-                        for (int j = 0; j < A.n_cand; j++) {
+                        for (int j = 0; j < nc; j++) {
                             cum += rdl(w, j);
                             if (cum < target) pick = j + 1;
                         }
-                        pick = pick < A.n_cand ? pick : A.n_cand - 1;
+                        pick = pick < nc ? pick : nc - 1;
                         bs = uint32_t(rdl_i(int(my_start), pick));
                     }
                     for (int r = 0; A.sel_temp <= 0.0f && r < A.top_k; r++) {
-                        const unsigned long long m = __ballot(ln < A.n_cand && rank == r);
+                        const unsigned long long m = __ballot(ln < nc && rank == r);
                         const int c = __ffsll((long long)m) - 1;
                         const uint32_t start = uint32_t(rdl_i(int(my_start), c));
                         Rng r2 = rng;
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
                     mutate(C, cand, r3);
                 } else {
                     // collect: every other candidate refined for its label
-                    for (int c = A.n_cand - 1; c >= 1; c--) {
+                    for (int c = nc - 1; c >= 1; c--) {
                         Rng r2 = rng;
                         r2.draw = uint32_t(rdl_i(int(my_start), c));
                         WPose<APL> t;
