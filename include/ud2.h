@@ -110,6 +110,29 @@ int ud2_scene_destroy(ud2_scene* sc);
  * Replaces the ligand half of DockTask::alloc_gpu (E/src/cuda/task.cu:16-284). */
 int ud2_ligset_create(const ud2_ligand_batch* batch, ud2_ligset** out);
 int ud2_ligset_destroy(ud2_ligset* ls);
+
+/* Bias sites and the bias option of the task: apply_bias (E/src/screening/core.cpp:86-97)
+ * plus the bias half of DockTask's ligand upload (FlexParamVina::inds_bias /
+ * params_bias, E/src/cuda/task.cu:157-168) and the BIAS_TYPE / BIAS_K constants
+ * (E/src/cuda/common.cu:25-26). bias_off_h [atoms+1] is a CSR over the set's
+ * atoms (atom_off order) into bias_h, 5 floats per site as in the input JSON
+ * (E/src/format/rapidjson_parser.cpp:114-124): x, y, z, V, r2. pos adds
+ * V k exp(-|site - atom|^2 / r2) per site (Vina::eval_ef_pos, E/src/score/vina.h:277-289);
+ * align adds the Z-align attraction with affinity V k (Vina::eval_ef_zalign,
+ * vina.h:306-324). The bias enters every search energy, uncurled, and is reported
+ * separately by ud2_final_score. NULL bias_off_h clears the sites. */
+#define UD2_BIAS_NO 0
+#define UD2_BIAS_POS 1
+#define UD2_BIAS_ALIGN 2
+int ud2_ligset_set_bias(ud2_ligset* ls, int32_t bias_type, float bias_k, const int32_t* bias_off_h,
+                        const float* bias_h);
+/* Constraint docking (DockParam::constraint_docking, FLAG_CONSTRAINT_DOCK of
+ * E/src/cuda/common.cu:27): the centre and orientation stay fixed in every
+ * search entry point, mutations draw torsions only (E/src/cuda/mc.cu:159-166),
+ * randomisation keeps the rigid pose (mc.cu:27-35), the rigid gradient
+ * components are zero (E/src/cuda/bfgs.cuh:176) and the pose update has no
+ * rigid step (bfgs.cuh:571). */
+int ud2_ligset_set_constraint(ud2_ligset* ls, int32_t constraint);
 /* Sizes of the pose buffers (floats) for nchain poses per ligand. */
 int ud2_ligset_pose_layout(const ud2_ligset* ls, int32_t nchain, int64_t* x_floats_h, int64_t* dihe_floats_h,
                            int64_t* n_pose_h);
@@ -150,10 +173,17 @@ int ud2_mc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const 
  * (chain indices), nleaders_d[n_lig]. */
 int ud2_cluster(const ud2_ligset* ls, const ud2_poses* P, float rmsd_limit, int32_t* leaders_d,
                 int32_t* nleaders_d, void* stream);
-/* Final re-score (score(), E/src/score/score.cpp:12-108): out3_d[n_sel][3] =
- * intra (pairs once, no curl), inter (direct sum, no curl), box penalty. */
+/* Final re-score (score(), E/src/score/score.cpp:12-108): out4_d[n_sel][4] =
+ * intra (pairs once, no curl), inter (direct sum, no curl), box penalty, bias. */
 int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
-                    int32_t n_sel, float* out3_d, void* stream);
+                    int32_t n_sel, float* out4_d, void* stream);
+/* Energy decomposition (score_decomp, E/src/score/score.cpp:115-147): the
+ * inter-molecular direct sum of each selected pose split per ligand atom into
+ * its weighted Vina terms, decomp_d[n_sel][atom_stride][5] = gauss1, gauss2,
+ * repulsion, hydrophobic, hbond (zero for hydrogens); atom_stride >= the
+ * largest natom among the selected poses. */
+int ud2_score_decomp(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                     int32_t n_sel, int32_t atom_stride, float* decomp_d, void* stream);
 
 /* ------------------------------------------------------------- learned proposal
  * No reference counterpart: the reference's MC tries one random mutation per

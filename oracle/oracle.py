@@ -25,7 +25,11 @@ _f32p = C.POINTER(C.c_float)
 class _Ligand(C.Structure):
     _fields_ = [("natom", C.c_int32), ("ntors", C.c_int32), ("npair", C.c_int32), ("types", _i32p),
                 ("axis", _i32p), ("rot_start", _i32p), ("rot_atoms", _i32p), ("range_start", _i32p),
-                ("ranges", _f32p), ("pairs", _i32p)]
+                ("ranges", _f32p), ("pairs", _i32p), ("bias_start", _i32p), ("bias", _f32p),
+                ("bias_type", C.c_int32), ("bias_k", C.c_float), ("constraint", C.c_int32)]
+
+
+BIAS_TYPES = {"no": 0, "pos": 1, "align": 2}
 
 
 class _Scene(C.Structure):
@@ -80,6 +84,7 @@ def _declare(L):
         getattr(L, f"ud2o_mc_{sfx}").restype = C.c_double
         getattr(L, f"ud2o_mc_{sfx}").argtypes = ([vp] * 6 + [C.c_int] * 7 + [C.c_uint32] + [vp] * 6 + [C.c_int])
         getattr(L, f"ud2o_final_score_{sfx}").argtypes = [vp, vp, vp, vp]
+        getattr(L, f"ud2o_score_decomp_{sfx}").argtypes = [vp, vp, vp, vp]
         getattr(L, f"ud2o_dock_ligand_{sfx}").restype = C.c_int
         getattr(L, f"ud2o_dock_ligand_{sfx}").argtypes = ([vp] * 6 + [C.c_int] * 6 + [C.c_uint32, C.c_float,
                                                            C.c_float, C.c_int] + [vp] * 4 + [C.c_int])
@@ -90,9 +95,14 @@ def _p(a: np.ndarray):
 
 
 class Topo:
-    """Keeps the contiguous arrays of one ligand alive and exposes the C struct."""
+    """Keeps the contiguous arrays of one ligand alive and exposes the C struct.
 
-    def __init__(self, lig):
+    The task options that change the energy or the search ride along: ``bias``
+    ("no", "pos", "align") with ``bias_k``, and ``constraint`` docking. Bias
+    sites come from the ligand's ``bias_start`` / ``bias`` arrays when it has them.
+    """
+
+    def __init__(self, lig, bias: str = "no", bias_k: float = 0.1, constraint: bool = False):
         self.types = np.ascontiguousarray(lig.types, dtype=np.int32)
         self.axis = np.ascontiguousarray(lig.axis, dtype=np.int32).reshape(-1)
         self.rot_start = np.ascontiguousarray(lig.rot_start, dtype=np.int32)
@@ -111,11 +121,18 @@ class Topo:
         if self.dihe0.size == 0:
             self.dihe0 = np.zeros(1, dtype=np.float32)
         self.natom, self.ntors, self.npair = int(lig.natom), int(lig.ntors), int(lig.npair)
+        bs = getattr(lig, "bias_start", None)
+        self.bias_start = (np.zeros(self.natom + 1, np.int32) if bs is None
+                           else np.ascontiguousarray(bs, dtype=np.int32))
+        bb = getattr(lig, "bias", None)
+        self.bias = np.ascontiguousarray(np.zeros(5, np.float32) if bb is None or np.size(bb) == 0 else bb,
+                                         dtype=np.float32).reshape(-1)
         self.s = _Ligand(self.natom, self.ntors, self.npair,
                          self.types.ctypes.data_as(_i32p), self.axis.ctypes.data_as(_i32p),
                          self.rot_start.ctypes.data_as(_i32p), self.rot_atoms.ctypes.data_as(_i32p),
                          self.range_start.ctypes.data_as(_i32p), self.ranges.ctypes.data_as(_f32p),
-                         self.pairs.ctypes.data_as(_i32p))
+                         self.pairs.ctypes.data_as(_i32p), self.bias_start.ctypes.data_as(_i32p),
+                         self.bias.ctypes.data_as(_f32p), BIAS_TYPES[bias], float(bias_k), int(bool(constraint)))
 
     @property
     def ref(self):
@@ -309,7 +326,7 @@ def energy_forces(topo: Topo, x, scene: Scene, v_cap=10.0, slope=1e6, sfx="f64")
     T = _dt(sfx)
     xx = np.ascontiguousarray(x, dtype=T).reshape(-1)
     f = np.zeros_like(xx)
-    terms = np.zeros(3, dtype=T)
+    terms = np.zeros(4, dtype=T)
     e = getattr(lib(), f"ud2o_energy_forces_{sfx}")(C.byref(topo.s), _p(xx), scene.ref, v_cap, slope, _p(f),
                                                      _p(terms))
     return e, f.reshape(-1, 3), terms
@@ -317,7 +334,7 @@ def energy_forces(topo: Topo, x, scene: Scene, v_cap=10.0, slope=1e6, sfx="f64")
 
 def energy_grad(topo: Topo, pose: PoseArr, scene: Scene, v_cap=10.0, slope=1e6, sfx="f64"):
     g = np.zeros(6 + topo.ntors, dtype=_dt(sfx))
-    terms = np.zeros(3, dtype=_dt(sfx))
+    terms = np.zeros(4, dtype=_dt(sfx))
     e = getattr(lib(), f"ud2o_energy_grad_{sfx}")(C.byref(topo.s), _p(pose.x), _p(pose.c), _p(pose.rv),
                                                    _p(pose.dihe), scene.ref, v_cap, slope, _p(g), _p(terms))
     return e, g, terms
@@ -408,10 +425,20 @@ def mc(topo: Topo, scene: Scene, start: PoseArr, lig_index: int, nchain: int, mc
 
 
 def final_score(topo: Topo, x, scene: Scene, sfx="f64") -> np.ndarray:
+    """[intra, inter, penalty, bias] of one conformation."""
     T = _dt(sfx)
     xx = np.ascontiguousarray(x, dtype=T).reshape(-1)
-    out = np.zeros(3, dtype=T)
+    out = np.zeros(4, dtype=T)
     getattr(lib(), f"ud2o_final_score_{sfx}")(C.byref(topo.s), _p(xx), scene.ref, _p(out))
+    return out
+
+
+def score_decomp(topo: Topo, x, scene: Scene, sfx="f64") -> np.ndarray:
+    """Per-atom inter-molecular terms [natom, 5]: gauss1, gauss2, repulsion, hydrophobic, hbond."""
+    T = _dt(sfx)
+    xx = np.ascontiguousarray(x, dtype=T).reshape(-1)
+    out = np.zeros((topo.natom, 5), dtype=T)
+    getattr(lib(), f"ud2o_score_decomp_{sfx}")(C.byref(topo.s), _p(xx), scene.ref, _p(out))
     return out
 
 

@@ -86,6 +86,17 @@ struct Ligand {
     const int32_t* range_start = nullptr; // [ntors+1] CSR in units of (lo, hi) pairs
     const float* ranges = nullptr;        // [2 * range_start[ntors]] radians
     const int32_t* pairs = nullptr;       // [npair*2] intra pairs (i, j)
+    // Bias sites (FlexParamVina::inds_bias / params_bias, E/src/cuda/task.cu:157-168):
+    // atom a owns sites bias_start[a] .. bias_start[a+1]-1, 5 floats each
+    // (x, y, z, V, r2); V is the well depth (pos) or the affinity a_qt (align).
+    const int32_t* bias_start = nullptr;  // [natom+1], or null: no sites
+    const float* bias = nullptr;          // [5 * bias_start[natom]]
+    // Task options of DockParam (E/src/model/model.h:151-168) that change the
+    // search: bias type (0 no, 1 pos, 2 align; BiasType, model.h:145-149), its
+    // scale bias_k, and constraint docking (no rigid degrees of freedom).
+    int bias_type = 0;
+    float bias_k = 0.1f;
+    int constraint = 0;
 };
 
 struct Box { float lo[3], hi[3]; };
@@ -125,7 +136,7 @@ struct Pose {
 };
 
 template <class T>
-struct Terms { T inter = 0, intra = 0, penalty = 0; };
+struct Terms { T inter = 0, intra = 0, penalty = 0, bias = 0; };
 
 // ------------------------------------------------------------- scalar math
 template <class T> inline T sq(T v) { return v * v; }
@@ -404,8 +415,10 @@ template <class T> void apply_torsion(const Ligand& L, Pose<T>& p, int t, T inc_
 // rigid step: centre += alpha*g_c, coordinates rotated about the OLD centre by
 // q(alpha*g_rot), rot_vec := alpha*g_rot. (clamp_to_center is the identity, mymath.h:112-119.)
 // g layout: [0..2] centre, [3..5] orientation, [6..6+ntors) torsions.
+// Constraint docking skips the rigid step (bfgs.cuh:571).
 template <class T> void apply_update(const Ligand& L, Pose<T>& p, const T* g, T alpha) {
     for (int t = 0; t < L.ntors; t++) apply_torsion(L, p, t, g[6 + t] * alpha);
+    if (L.constraint) return;
     T c_old[3] = {p.c[0], p.c[1], p.c[2]};
     p.c[0] = c_old[0] + alpha * g[0];
     p.c[1] = c_old[1] + alpha * g[1];
@@ -476,6 +489,48 @@ template <class T> T grid_point_value(const Receptor& R, int probe_type, const T
     return s;
 }
 
+// Vina::eval_ef_pos, vina.h:277-289: Gaussian well E = v exp(-|r|^2 / r2) with
+// r = site - atom; f += dE/d(atom).
+template <class T> inline T bias_pos(const T* r, T v, T r2, T* f) {
+    T e = v * std::exp(-dot3(r, r) / r2);
+    for (int k = 0; k < 3; k++) f[k] += e * T(2) * r[k] / r2;
+    return e;
+}
+
+// Vina::eval_ef_zalign, vina.h:306-324 (Wang et al., JCIM 64:6205, 2024): the
+// attraction E = a k_a / (1 + exp(s_a (c_a - |r|))), s_a = 3, c_a = 1, k_a = 0.1
+// for hydrogen and 30 otherwise; f += dE/d(atom). |r| = 0 divides by zero, as
+// in the reference.
+template <class T> inline T bias_zalign(const T* r, T a, int type, T* f) {
+    const T s_a = 3, c_a = 1, k_a = type == XS_H ? T(0.1) : T(30);
+    T d = norm3(r);
+    T ei = std::exp(s_a * (c_a - d));
+    T e = a * k_a / (T(1) + ei);
+    T fd = -e * ei / (T(1) + ei) * s_a;
+    for (int k = 0; k < 3; k++) f[k] += fd * r[k] / d;
+    return e;
+}
+
+// Bias energy of a conformation, the bias block of cal_e_f_tile (bfgs.cuh:355-388)
+// and of score() (score.cpp:85-107): every site of every atom (hydrogens
+// included), no curl. Adds dE/dx to f when f is given.
+template <class T> T bias_energy(const Ligand& L, const T* x, T* f) {
+    if (L.bias_type == 0 || !L.bias_start) return T(0);
+    T e = 0;
+    for (int a = 0; a < L.natom; a++) {
+        T fa[3] = {0, 0, 0};
+        for (int j = L.bias_start[a]; j < L.bias_start[a + 1]; j++) {
+            const float* b = L.bias + 5 * j;
+            T r[3] = {T(b[0]) - x[3 * a], T(b[1]) - x[3 * a + 1], T(b[2]) - x[3 * a + 2]};
+            T vk = T(b[3]) * T(L.bias_k);
+            if (L.bias_type == 1) e += bias_pos(r, vk, T(b[4]), fa);
+            else if (L.bias_type == 2) e += bias_zalign(r, vk, L.types[a], fa);
+        }
+        if (f) { f[3 * a] += fa[0]; f[3 * a + 1] += fa[1]; f[3 * a + 2] += fa[2]; }
+    }
+    return e;
+}
+
 // cal_box_penalty, bfgs.cuh:196-218.
 template <class T> T box_penalty(const T* x, const Box& b, T* f, T slope) {
     T pen = 0;
@@ -489,8 +544,7 @@ template <class T> T box_penalty(const T* x, const Box& b, T* f, T slope) {
 // cal_e_f_tile, bfgs.cuh:221-394: total energy under the curl cap v_cap, and
 // f = dE/dx per atom. Inter: direct sum (:242-290) or grid map; per-atom curl
 // (:276-283). Penalty on heavy atoms (:293-300). Intra with per-pair curl
-// (:305-353), each pair once (see the header). Bias terms are not modelled
-// (the reference's default bias is "no", E/src/screening/core.h:13).
+// (:305-353), each pair once (see the header). Bias sites (:355-388), uncurled.
 template <class T>
 T energy_forces(const Ligand& L, const T* x, const Scene& S, T v_cap, T slope, T* f, Terms<T>* terms = nullptr) {
     std::fill(f, f + 3 * L.natom, T(0));
@@ -552,8 +606,9 @@ T energy_forces(const Ligand& L, const T* x, const Scene& S, T v_cap, T slope, T
             f[3 * j] -= fp[0]; f[3 * j + 1] -= fp[1]; f[3 * j + 2] -= fp[2];
         }
     }
-    if (terms) { terms->inter = e_inter; terms->intra = e_intra; terms->penalty = e_pen; }
-    return e_inter + e_pen + e_intra;
+    T e_bias = bias_energy(L, x, f);
+    if (terms) { terms->inter = e_inter; terms->intra = e_intra; terms->penalty = e_pen; terms->bias = e_bias; }
+    return e_inter + e_pen + e_intra + e_bias;
 }
 
 // cal_grad_tile, bfgs.cuh:69-193: torsion torques projected on the unit axis,
@@ -595,6 +650,10 @@ template <class T> void pose_gradient(const Ligand& L, const Pose<T>& p, const T
         for (int j = 0; j < 9; j++) s += mat[j] * dR[j];
         g[3 + k] = s;
     }
+    // constraint docking (bfgs.cuh:176): the rigid components are never written
+    // and keep the zeros the gradient buffers start with
+    if (L.constraint)
+        for (int k = 0; k < 6; k++) g[k] = 0;
 }
 
 // cal_e_grad_tile, bfgs.cuh:397-410.
@@ -721,16 +780,20 @@ template <class T> void super_fibonacci_rotvec(int idx, int n, T* v) {
     quat_to_rotvec(q, v);
 }
 
-// randomize_pose_tile, mc.cu:18-104 (the non-constraint, non-align branch):
-// centre drawn in the box, super-Fibonacci orientation by chain index, torsions
-// drawn in a random allowed range; applied through apply_update with alpha = 1.
-// Draws: one block for the centre, then one block per torsion.
+// randomize_pose_tile, mc.cu:18-104: centre drawn in the box, super-Fibonacci
+// orientation by chain index, torsions drawn in a random allowed range; applied
+// through apply_update with alpha = 1. Constraint docking keeps the centre and
+// the orientation (:27-35), the align bias keeps the centre (:36-39).
+// Draws: one block for the centre (also when it is kept), then one block per torsion.
 template <class T> void randomize(const Ligand& L, const Box& box, Pose<T>& p, int idx, int n, ChainRng& rng) {
     std::vector<T> g(6 + L.ntors, T(0));
     uint32_t b[4];
     rng.next(b);
-    for (int d = 0; d < 3; d++) g[d] = (T(box.lo[d]) + (T(box.hi[d]) - T(box.lo[d])) * T(u01(b[d]))) - p.c[d];
-    super_fibonacci_rotvec<T>(idx, n, &g[3]);
+    if (!L.constraint) {
+        if (L.bias_type != 2)
+            for (int d = 0; d < 3; d++) g[d] = (T(box.lo[d]) + (T(box.hi[d]) - T(box.lo[d])) * T(u01(b[d]))) - p.c[d];
+        super_fibonacci_rotvec<T>(idx, n, &g[3]);
+    }
     for (int t = 0; t < L.ntors; t++) {
         rng.next(b);
         int r0 = L.range_start[t], nr = L.range_start[t + 1] - r0;
@@ -757,11 +820,18 @@ inline void rand_in_ball(ChainRng& rng, float out[3]) {
 }
 
 // mutate_pose_tile, mc.cu:146-256: one of translation, rigid rotation, or one
-// torsion set to a value drawn in its allowed ranges.
+// torsion set to a value drawn in its allowed ranges. Constraint docking draws
+// a torsion only, and a rigid ligand then does not move (:159-166).
 template <class T> void mutate(const Ligand& L, const Box& box, Pose<T>& p, ChainRng& rng) {
     uint32_t b[4];
     rng.next(b);
-    int which = int(b[0] % uint32_t(2 + L.ntors));
+    int which;
+    if (L.constraint) {
+        if (L.ntors == 0) return;
+        which = 2 + int(b[0] % uint32_t(L.ntors));
+    } else {
+        which = int(b[0] % uint32_t(2 + L.ntors));
+    }
     if (which == 0) {
         float r[3];
         rand_in_ball(rng, r);
@@ -918,8 +988,9 @@ inline std::vector<int> filter_poses(const std::vector<int>& clustered, const fl
 }
 
 // score(), E/src/score/score.cpp:12-108: final re-score with no curl, intra
-// pairs once, direct inter sum, penalty at the fixed slope. out = [intra, inter, penalty].
-template <class T> void final_score(const Ligand& L, const T* x, const Receptor& R, const Box& box, T out[3]) {
+// pairs once, direct inter sum, penalty at the fixed slope, bias sites.
+// out = [intra, inter, penalty, bias].
+template <class T> void final_score(const Ligand& L, const T* x, const Receptor& R, const Box& box, T out[4]) {
     T e_intra = 0, e_inter = 0, e_pen = 0, dd;
     for (int k = 0; k < L.npair; k++) {
         int i = L.pairs[2 * k], j = L.pairs[2 * k + 1];
@@ -949,6 +1020,29 @@ template <class T> void final_score(const Ligand& L, const T* x, const Receptor&
     for (int i = 0; i < L.natom; i++)
         if (L.types[i] != XS_H) e_pen += box_penalty(x + 3 * i, box, ftmp, T(kPenaltySlope));
     out[0] = e_intra; out[1] = e_inter; out[2] = e_pen;
+    out[3] = bias_energy<T>(L, x, nullptr);
+}
+
+// score_decomp, E/src/score/score.cpp:115-147: the inter-molecular direct sum
+// split per ligand atom into its five weighted terms (gauss1, gauss2,
+// repulsion, hydrophobic, hbond); out[natom][5], zero for hydrogens.
+template <class T> void score_decomp(const Ligand& L, const T* x, const Receptor& R, T* out) {
+    std::fill(out, out + 5 * L.natom, T(0));
+    for (int i = 0; i < L.natom; i++) {
+        int ti = L.types[i];
+        if (ti == XS_H) continue;
+        for (int j = 0; j < R.natom; j++) {
+            int tj = R.types[j];
+            if (tj == XS_H) continue;
+            T r[3] = {T(R.xyz[3 * j]) - x[3 * i], T(R.xyz[3 * j + 1]) - x[3 * i + 1], T(R.xyz[3 * j + 2]) - x[3 * i + 2]};
+            T rr = dot3(r, r);
+            if (rr < T(kR2Cutoff)) {
+                T t5[5];
+                eval_decomp(std::sqrt(rr) - (T(kVdwRadius[ti]) + T(kVdwRadius[tj])), ti, tj, t5);
+                for (int k = 0; k < 5; k++) out[5 * i + k] += t5[k];
+            }
+        }
+    }
 }
 
 // vina_conf_indep, vina.h:202-204.

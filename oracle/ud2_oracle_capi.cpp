@@ -23,6 +23,11 @@ typedef struct {
     const int32_t* range_start;
     const float* ranges;
     const int32_t* pairs;
+    const int32_t* bias_start;  // [natom+1] or null
+    const float* bias;          // [5 * sites]: x, y, z, V, r2
+    int32_t bias_type;          // 0 no, 1 pos, 2 align
+    float bias_k;
+    int32_t constraint;         // 1: constraint docking
 } ud2o_ligand;
 
 typedef struct {
@@ -47,6 +52,8 @@ Ligand to_lig(const ud2o_ligand* l) {
     L.natom = l->natom; L.ntors = l->ntors; L.npair = l->npair;
     L.types = l->types; L.axis = l->axis; L.rot_start = l->rot_start; L.rot_atoms = l->rot_atoms;
     L.range_start = l->range_start; L.ranges = l->ranges; L.pairs = l->pairs;
+    L.bias_start = l->bias_start; L.bias = l->bias;
+    L.bias_type = l->bias_type; L.bias_k = l->bias_k; L.constraint = l->constraint;
     return L;
 }
 
@@ -226,23 +233,23 @@ void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_ty
 
 #define DEF_NUMERIC(T, SFX)                                                                                         \
     T ud2o_energy_forces_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T v_cap, T slope, T* f,     \
-                               T* terms3) {                                                                        \
+                               T* terms4) {                                                                        \
         Ligand L = to_lig(l);                                                                                       \
         SceneHolder S(s);                                                                                           \
         Terms<T> tm;                                                                                                \
         T e = energy_forces<T>(L, x, S.scene, v_cap, slope, f, &tm);                                               \
-        if (terms3) { terms3[0] = tm.inter; terms3[1] = tm.intra; terms3[2] = tm.penalty; }                        \
+        if (terms4) { terms4[0] = tm.inter; terms4[1] = tm.intra; terms4[2] = tm.penalty; terms4[3] = tm.bias; }   \
         return e;                                                                                                   \
     }                                                                                                               \
     T ud2o_energy_grad_##SFX(const ud2o_ligand* l, const T* x, const T* c, const T* rv, const T* dihe,             \
-                             const ud2o_scene* s, T v_cap, T slope, T* g, T* terms3) {                             \
+                             const ud2o_scene* s, T v_cap, T slope, T* g, T* terms4) {                             \
         Ligand L = to_lig(l);                                                                                       \
         SceneHolder S(s);                                                                                           \
         Pose<T> p = load_pose<T>(L, x, c, rv, dihe, T(0));                                                          \
         std::vector<T> fb;                                                                                          \
         Terms<T> tm;                                                                                                \
         T e = energy_grad<T>(L, p, S.scene, v_cap, slope, g, fb, &tm);                                             \
-        if (terms3) { terms3[0] = tm.inter; terms3[1] = tm.intra; terms3[2] = tm.penalty; }                        \
+        if (terms4) { terms4[0] = tm.inter; terms4[1] = tm.intra; terms4[2] = tm.penalty; terms4[3] = tm.bias; }   \
         return e;                                                                                                   \
     }                                                                                                               \
     void ud2o_apply_torsion_##SFX(const ud2o_ligand* l, T* x, T* c, T* rv, T* dihe, int t, T inc) {                \
@@ -343,10 +350,15 @@ void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_ty
         if (out_evals) { long long s_ = 0; for (auto v : ev) s_ += v; *out_evals = s_; }                            \
         return dt;                                                                                                  \
     }                                                                                                               \
-    void ud2o_final_score_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T* out3) {                  \
+    void ud2o_final_score_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T* out4) {                  \
         Ligand L = to_lig(l);                                                                                       \
         SceneHolder S(s);                                                                                           \
-        final_score<T>(L, x, S.rec, S.scene.box, out3);                                                             \
+        final_score<T>(L, x, S.rec, S.scene.box, out4);                                                             \
+    }                                                                                                               \
+    void ud2o_score_decomp_##SFX(const ud2o_ligand* l, const T* x, const ud2o_scene* s, T* out) {                  \
+        Ligand L = to_lig(l);                                                                                       \
+        SceneHolder S(s);                                                                                           \
+        score_decomp<T>(L, x, S.rec, out);                                                                          \
     }                                                                                                               \
     /* Whole docking task of one ligand, DockTask::run (E/src/task/DockTask.cpp:39-123): MC chains,                \
        clustering, refinement with the slope ramp (:155-186), filter (:188-220) and final scoring with             \
@@ -387,7 +399,7 @@ void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_ty
         }                                                                                                           \
         for (int k = 0; k < nchain; k++) ef[k] = float(res[k].energy);                                              \
         std::vector<int> keep = filter_poses(leaders, ef.data(), energy_range, num_pose);                           \
-        T sc[3];                                                                                                    \
+        T sc[4];                                                                                                    \
         T intra_r1 = 0;                                                                                             \
         for (size_t k = 0; k < keep.size(); k++) {                                                                  \
             const Pose<T>& p = res[keep[k]];                                                                        \
@@ -398,7 +410,7 @@ void ud2o_grid_points(int rec_natom, const float* rec_xyz, const int32_t* rec_ty
             T aff = conf_indep<T>(inter, L.ntors);                                                                  \
             T* e7 = out_e + 7 * k;                                                                                  \
             e7[0] = aff; e7[1] = total; e7[2] = sc[0]; e7[3] = sc[1]; e7[4] = sc[2]; e7[5] = aff - inter;          \
-            e7[6] = 0;                                                                                              \
+            e7[6] = sc[3];                                                                                          \
             std::copy(p.x.begin(), p.x.end(), out_x + k * 3 * L.natom);                                             \
         }                                                                                                           \
         if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();      \

@@ -264,7 +264,7 @@ def test_final_score_matches_oracle(gpu, scene, oracle_scenes, lig_c2, box):
         p = O.pose_from_vec(topo, pv[c].astype(np.float64))
         pb.set_pose(0, c, p.x, p.c, p.rv, p.dihe[:lig_c2.ntors])
     sel = torch.arange(nchain, dtype=torch.int32, device="cuda")
-    out = torch.zeros((nchain, 3), device="cuda")
+    out = torch.zeros((nchain, 4), device="cuda")
     gpu.check(gpu.lib().ud2_final_score(scene.h, ls.h, pb.ref, dptr(sel), nchain, dptr(out), gpu.stream_ptr()))
     torch.cuda.synchronize()
     for c in range(nchain):

@@ -40,7 +40,8 @@ def test_dock_batch_matches_oracle(gpu, receptor, box):
             topo = O.Topo(lig)
             for k in range(len(r.coords)):
                 ref = O.final_score(topo, r.coords[k], dsc)
-                assert np.abs(E[k, 2:5] - ref).max() <= 1e-4 * max(1.0, np.abs(ref).max())
+                assert np.abs(E[k, 2:5] - ref[:3]).max() <= 1e-4 * max(1.0, np.abs(ref).max())
+                assert E[k, 6] == 0.0 and ref[3] == 0.0  # no bias sites
             aff = (E[:, 1] - E[0, 2]) / (1 + 0.05846 * lig.ntors)
             assert np.allclose(E[:, 0], aff, atol=1e-4)
             ox, oe, _, _ = O.dock_ligand(topo, osc, O.PoseArr.initial(lig), i, 24, 6, 10, 3, seed, 1.0, 10.0, 5,

@@ -1,4 +1,6 @@
 """Host-side logic of the drop-in pipeline (no GPU): presets, grouping, output format."""
+import json
+
 import numpy as np
 import pytest
 
@@ -83,6 +85,20 @@ def test_write_retries(tmp_path):
 
 
 def test_unsupported_options_fail_loudly():
-    pipe = P.DockingPipeline(center_x=0, center_y=0, center_z=0, size_x=10, size_y=10, size_z=10, bias="pos")
-    with pytest.raises(RuntimeError, match="Core pipeline failed"):
+    """apply_bias (core.cpp:86-97) rejects an unknown bias type before any GPU work."""
+    pipe = P.DockingPipeline(center_x=0, center_y=0, center_z=0, size_x=10, size_y=10, size_z=10, bias="hbond")
+    with pytest.raises(RuntimeError, match="Core pipeline failed") as ei:
         pipe.run()
+    assert "Not supported bias" in str(ei.value.__cause__)
+
+
+def test_decomp_member_in_output_document():
+    """json.cpp:134-147: "decomp" follows "dihedrals", one [5] row per atom."""
+    from unidock_engine.engine import LigandResult
+    r = LigandResult("m", coords=[np.zeros(6, np.float32)], dihedrals=[np.zeros(0, np.float32)],
+                     energies=[np.arange(7, dtype=np.float32)],
+                     decomp=[np.asarray([[-0.5, -0.25, 0.125, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 0.0]], np.float32)])
+    doc = json.loads(P.dumps_poses([r]))
+    pose = doc["m"][0]
+    assert list(pose) == ["energy", "coords", "dihedrals", "decomp"]
+    assert pose["decomp"] == [[-0.5, -0.25, 0.125, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 0.0]]

@@ -94,3 +94,19 @@ def test_synthetic_receptor():
     dd = np.linalg.norm(rec.xyz[:, None] - rec.xyz[None], axis=-1) + np.eye(400) * 10
     assert dd.min() >= 3.0 - 1e-4
     assert set(np.unique(rec.types)) <= {2, 3, 4, 5, 6, 10, 11, 15}
+
+
+def test_bias_sites_of_the_bias_hbond_fixture():
+    """E/test/data/bias_hbond/input.json: atom 6 carries the one site [1.782, 66.634, 6.337, -15, 1.0]
+    (E/test/feature/bias_hbond/validate.py:2); parsed as the ninth atom field (rapidjson_parser.cpp:113-124)."""
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    doc = json.load(open(os.path.join(here, "golden", "bias_hbond_input.json")))
+    lig = T.ligand_from_json("MOL_0", doc["MOL_0"])
+    assert lig.natom == 41 and lig.bias_start is not None
+    counts = np.diff(lig.bias_start)
+    assert counts[6] == 1 and counts.sum() == 1
+    assert np.allclose(lig.bias[0], [1.782, 66.634, 6.337, -15.0, 1.0])
+    plain_atoms = [a[:8] for a in doc["MOL_0"]["atoms"]]  # the same ligand without its site
+    assert T.ligand_from_json("m", dict(doc["MOL_0"], atoms=plain_atoms)).bias_start is None

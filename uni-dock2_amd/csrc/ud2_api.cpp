@@ -113,6 +113,12 @@ struct ud2_ligset {
     int32_t* d_order = nullptr;   // work order, n_lig entries
     int32_t* d_sel_all = nullptr; // scratch for all-pose selections
     int64_t sel_all_cap = 0;
+    int32_t* d_bias_start = nullptr;  // [atoms + 1], null without sites
+    float4* d_bias_site = nullptr;
+    float* d_bias_r2 = nullptr;
+    int32_t bias_type = 0;
+    float bias_k = 0.1f;
+    int32_t constraint = 0;
 
     LigSetDev view() const {
         LigSetDev v;
@@ -124,6 +130,12 @@ struct ud2_ligset {
         v.ranges = d_ranges;
         v.tors_mask = d_mask;
         v.pairs = d_pairs;
+        v.bias_start = d_bias_start;
+        v.bias_site = d_bias_site;
+        v.bias_r2 = d_bias_r2;
+        v.bias_type = bias_type;
+        v.bias_k = bias_k;
+        v.constraint = constraint;
         return v;
     }
     int max_apl() const {
@@ -538,11 +550,62 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
 
 int ud2_ligset_destroy(ud2_ligset* ls) {
     if (!ls) return UD2_OK;
-    void* ptrs[] = {ls->d_ligs, ls->d_x0, ls->d_axis, ls->d_dihe0, ls->d_trange, ls->d_ranges,
-                    ls->d_mask, ls->d_pairs, ls->d_counter, ls->d_order, ls->d_sel_all};
+    void* ptrs[] = {ls->d_ligs,   ls->d_x0,      ls->d_axis,    ls->d_dihe0,       ls->d_trange,
+                    ls->d_ranges, ls->d_mask,    ls->d_pairs,   ls->d_counter,     ls->d_order,
+                    ls->d_sel_all, ls->d_bias_start, ls->d_bias_site, ls->d_bias_r2};
     for (void* p : ptrs)
         if (p) hipFree(p);
     delete ls;
+    return UD2_OK;
+}
+
+int ud2_ligset_set_bias(ud2_ligset* ls, int32_t bias_type, float bias_k, const int32_t* bias_off_h,
+                        const float* bias_h) {
+    if (!ls) return fail(UD2_ERR_ARG, "null ligand set");
+    if (bias_type < UD2_BIAS_NO || bias_type > UD2_BIAS_ALIGN)
+        return fail(UD2_ERR_ARG, "bias_type must be 0 (no), 1 (pos) or 2 (align)");
+    if (!std::isfinite(bias_k)) return fail(UD2_ERR_ARG, "non-finite bias_k");
+    const int natoms = ls->h_ligs.back().atom_off + ls->h_ligs.back().natom;
+    std::vector<int32_t> start;
+    std::vector<float4> site;
+    std::vector<float> r2;
+    if (bias_off_h) {
+        if (bias_off_h[0] != 0) return fail(UD2_ERR_ARG, "bias offsets must start at 0");
+        for (int a = 0; a < natoms; a++)
+            if (bias_off_h[a + 1] < bias_off_h[a]) return fail(UD2_ERR_ARG, "bias offsets must not decrease");
+        const int nsite = bias_off_h[natoms];
+        if (nsite > 0 && !bias_h) return fail(UD2_ERR_ARG, "null bias array");
+        for (int j = 0; j < nsite; j++) {
+            const float* b = bias_h + 5 * j;
+            for (int k = 0; k < 5; k++)
+                if (!std::isfinite(b[k])) return fail(UD2_ERR_ARG, "non-finite bias parameter at site " + std::to_string(j));
+            if (bias_type == UD2_BIAS_POS && !(b[4] > 0.0f))
+                return fail(UD2_ERR_ARG, "pos bias site " + std::to_string(j) + " needs r2 > 0");
+            site.push_back(make_float4(b[0], b[1], b[2], b[3]));
+            r2.push_back(b[4]);
+        }
+        if (nsite > 0) start.assign(bias_off_h, bias_off_h + natoms + 1);
+    }
+    void* old[] = {ls->d_bias_start, ls->d_bias_site, ls->d_bias_r2};
+    for (void* p : old)
+        if (p) UD2_HIP(hipFree(p));
+    ls->d_bias_start = nullptr;
+    ls->d_bias_site = nullptr;
+    ls->d_bias_r2 = nullptr;
+    if (!start.empty()) {
+        int rc = upload(&ls->d_bias_start, start);
+        if (!rc) rc = upload(&ls->d_bias_site, site);
+        if (!rc) rc = upload(&ls->d_bias_r2, r2);
+        if (rc) return rc;
+    }
+    ls->bias_type = bias_type;
+    ls->bias_k = bias_k;
+    return UD2_OK;
+}
+
+int ud2_ligset_set_constraint(ud2_ligset* ls, int32_t constraint) {
+    if (!ls) return fail(UD2_ERR_ARG, "null ligand set");
+    ls->constraint = constraint ? 1 : 0;
     return UD2_OK;
 }
 
@@ -726,20 +789,21 @@ int ud2_cluster(const ud2_ligset* ls, const ud2_poses* P, float rmsd_limit, int3
     return UD2_OK;
 }
 
-int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
-                    int32_t n_sel, float* out3_d, void* stream) {
-    int rc = check_poses(ls, P);
-    if (rc) return rc;
-    if (!sc) return fail(UD2_ERR_ARG, "null scene");
-    if (n_sel <= 0) return UD2_OK;
-    if (!sel_d || !out3_d) return fail(UD2_ERR_ARG, "null buffer");
+namespace {
+
+int run_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d, int32_t n_sel,
+              float* out4_d, float* decomp_d, int32_t decomp_stride, void* stream) {
     std::vector<int32_t> h(n_sel);
     UD2_HIP(hipMemcpy(h.data(), sel_d, size_t(n_sel) * sizeof(int32_t), hipMemcpyDeviceToHost));
     int apl_max = 1;
     const int64_t npose = int64_t(ls->n_lig) * P->nchain;
     for (int32_t p : h) {
         if (p < 0 || p >= npose) return fail(UD2_ERR_ARG, "pose id out of range: " + std::to_string(p));
-        apl_max = std::max(apl_max, int(ls->h_ligs[p / P->nchain].apl));
+        const DevLig& d = ls->h_ligs[p / P->nchain];
+        if (decomp_d && d.natom > decomp_stride)
+            return fail(UD2_ERR_ARG, "atom_stride " + std::to_string(decomp_stride) + " is below the " +
+                                         std::to_string(d.natom) + " atoms of pose " + std::to_string(p));
+        apl_max = std::max(apl_max, int(d.apl));
     }
     ScoreArgs A;
     A.L = ls->view();
@@ -748,10 +812,35 @@ int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* 
     A.nchain = P->nchain;
     A.sel = sel_d;
     A.n_sel = n_sel;
-    A.out3 = out3_d;
+    A.out4 = out4_d;
+    A.decomp = decomp_d;
+    A.decomp_stride = decomp_stride;
     // one launch at the widest class: narrower ligands run with idle register slots
-    UD2_HIP(launch_final_score(A, apl_max, static_cast<hipStream_t>(stream)));
+    UD2_HIP(launch_final_score(A, apl_max, decomp_d != nullptr, static_cast<hipStream_t>(stream)));
     return UD2_OK;
+}
+
+}  // namespace
+
+int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                    int32_t n_sel, float* out4_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    if (n_sel <= 0) return UD2_OK;
+    if (!sel_d || !out4_d) return fail(UD2_ERR_ARG, "null buffer");
+    return run_score(sc, ls, P, sel_d, n_sel, out4_d, nullptr, 0, stream);
+}
+
+int ud2_score_decomp(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
+                     int32_t n_sel, int32_t atom_stride, float* decomp_d, void* stream) {
+    int rc = check_poses(ls, P);
+    if (rc) return rc;
+    if (!sc) return fail(UD2_ERR_ARG, "null scene");
+    if (n_sel <= 0) return UD2_OK;
+    if (!sel_d || !decomp_d) return fail(UD2_ERR_ARG, "null buffer");
+    if (atom_stride <= 0) return fail(UD2_ERR_ARG, "atom_stride must be positive");
+    return run_score(sc, ls, P, sel_d, n_sel, nullptr, decomp_d, atom_stride, stream);
 }
 
 }  // extern "C"

@@ -264,7 +264,7 @@ struct WPose {
 template <int APL>
 struct Ctx {
     int natom, ntors, npair, nheavy, dim;
-    int tors_off, pair_off;
+    int atom_off, tors_off, pair_off;
     int type[APL];      // XS type; -1 for lanes beyond natom
     int slot[APL];      // grid map of the atom's type
     int lx, lf, lo, lh, ld;
@@ -280,7 +280,7 @@ __device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const 
     const DevLig d = L.ligs[lig];
     C.natom = d.natom; C.ntors = d.ntors; C.npair = d.npair; C.nheavy = d.nheavy;
     C.dim = 6 + d.ntors;
-    C.tors_off = d.tors_off; C.pair_off = d.pair_off;
+    C.atom_off = d.atom_off; C.tors_off = d.tors_off; C.pair_off = d.pair_off;
     C.L = &L; C.S = &S; C.mode = mode;
     C.evals = 0;
     C.lx = base;
@@ -445,6 +445,7 @@ __device__ __forceinline__ void apply_update(const Ctx<APL>& C, WPose<APL>& P, f
 #pragma unroll 1
     for (int t = 0; t < C.ntors; t++) apply_torsion(C, P, t, rdl(dir, 6 + t) * alpha);
     if (C.ntors > 0) centroid(C, P);
+    if (C.L->constraint) return;  // constraint docking: no rigid step (bfgs.cuh:571)
     const float o0 = P.c0, o1 = P.c1, o2 = P.c2;
     P.c0 = uni(o0 + alpha * rdl(dir, 0));
     P.c1 = uni(o1 + alpha * rdl(dir, 1));
@@ -461,10 +462,54 @@ struct Forces {
     float fx[APL], fy[APL], fz[APL];
 };
 
+// Energy of bias site b (x, y, z, V) at atom position p, and s with dE/dp = s * r,
+// r = site - p. pos: Gaussian well V k exp(-|r|^2 / r2) (Vina::eval_ef_pos,
+// E/src/score/vina.h:277-289). align: a k k_a / (1 + exp(3 (1 - |r|))), k_a = 0.1 for
+// hydrogen, else 30 (Vina::eval_ef_zalign, vina.h:306-324; |r| = 0 divides by zero
+// as there).
+__device__ __forceinline__ float bias_site_energy(int bias_type, float bias_k, float4 b, float r2, int type,
+                                                  float rx, float ry, float rz, float& s) {
+    const float v = b.w * bias_k;
+    const float rr = rx * rx + ry * ry + rz * rz;
+    if (bias_type == 1) {
+        const float e = v * __expf(-rr / r2);
+        s = e * 2.0f / r2;
+        return e;
+    }
+    const float d = sqrtf(rr);
+    const float ka = type == kTypeH ? 0.1f : 30.0f;
+    const float ei = __expf(3.0f * (1.0f - d));
+    const float e = v * ka / (1.0f + ei);
+    s = -e * ei / (1.0f + ei) * 3.0f / d;
+    return e;
+}
+
+// Bias sites of every atom (hydrogens included), uncurled (the bias block of
+// cal_e_f_tile, bfgs.cuh:355-388): returns this lane's energy and adds dE/dx to F.
+template <int APL>
+__device__ __forceinline__ float bias_forces(const Ctx<APL>& C, const WPose<APL>& P, Forces<APL>& F) {
+    const LigSetDev& L = *C.L;
+    float e = 0.0f;
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        if (!valid(C, k)) continue;
+        const int a = C.atom_off + lane() + kWave * k;
+        const int j1 = L.bias_start[a + 1];
+        for (int j = L.bias_start[a]; j < j1; j++) {
+            const float4 b = L.bias_site[j];
+            const float rx = b.x - P.x[k], ry = b.y - P.y[k], rz = b.z - P.z[k];
+            float s;
+            e += bias_site_energy(L.bias_type, L.bias_k, b, L.bias_r2[j], C.type[k], rx, ry, rz, s);
+            F.fx[k] += s * rx; F.fy[k] += s * ry; F.fz[k] += s * rz;
+        }
+    }
+    return e;
+}
+
 // cal_e_f_tile (bfgs.cuh:221-394): energy under the curl cap v_cap and the
 // per-atom gradient F.
 template <int APL>
-__device__ float energy_forces(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope, Forces<APL>& F) {
+__device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<APL>& P, float v_cap, float slope, Forces<APL>& F) {
     const SceneDev& S = *C.S;
     const int ln = lane();
     const int n3 = kWave * APL;
@@ -566,13 +611,14 @@ __device__ float energy_forces(const Ctx<APL>& C, const WPose<APL>& P, float v_c
         fy[k] += s_lds[C.lf + n3 + a];
         fz[k] += s_lds[C.lf + 2 * n3 + a];
     }
+    if (C.L->bias_type != 0 && C.L->bias_start) e += bias_forces(C, P, F);
     return uni(wave_sum(e));
 }
 
 // cal_grad_tile (bfgs.cuh:69-193): the pose gradient from the per-atom gradient,
 // returned lane-distributed (lane k = component k).
 template <int APL>
-__device__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& P, const Forces<APL>& F) {
+__device__ __forceinline__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& P, const Forces<APL>& F) {
     const int ln = lane();
     const float* fx = F.fx;
     const float* fy = F.fy;
@@ -621,6 +667,9 @@ __device__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& P, const Forces<
         v = wave_sum(st);
         gout = ln == 6 + t ? v : gout;
     }
+    // constraint docking (bfgs.cuh:176): the rigid components are never written and
+    // keep the zeros the reference's gradient buffers start with
+    if (C.L->constraint && ln < 6) gout = 0.0f;
     return gout;
 }
 
@@ -636,7 +685,7 @@ __device__ __forceinline__ float energy_grad(const Ctx<APL>& C, const WPose<APL>
 
 // line_search_tile (bfgs.cuh:623-670).
 template <int APL>
-__device__ void line_search(const Ctx<APL>& C, const WPose<APL>& x, float g, float p, float e0, float v_cap,
+__device__ __forceinline__ void line_search(const Ctx<APL>& C, const WPose<APL>& x, float g, float p, float e0, float v_cap,
                             float slope, WPose<APL>& x_new, float& g_new, float& e_out, float& alpha_out) {
     const float pg = uni(wave_sum(p * g));
     float alpha = 1.0f, e_new = 0.0f;
@@ -658,7 +707,7 @@ __device__ void line_search(const Ctx<APL>& C, const WPose<APL>& x, float g, flo
 // tile (lane i owns row i) and the start pose is parked in LDS for the
 // restore-on-failure at the end.
 template <int APL>
-__device__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_cap, float slope) {
+__device__ __forceinline__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_cap, float slope) {
     const int ln = lane(), dim = C.dim;
     const int hrow = C.lh + ln * C.ld;
     const bool row = ln < dim;
@@ -777,12 +826,19 @@ struct Move {
 
 // mutate_pose_tile (E/src/cuda/mc.cu:146-256).
 template <int APL>
-__device__ Move mutate(const Ctx<APL>& C, WPose<APL>& P, Rng& rng) {
+__device__ __forceinline__ Move mutate(const Ctx<APL>& C, WPose<APL>& P, Rng& rng) {
     const SceneDev& S = *C.S;
     const uint4 b = rng.next();
-    const int which = int(b.x % uint32_t(2 + C.ntors));
+    // constraint docking draws a torsion only; a rigid ligand then stays put (mc.cu:159-166)
+    int which;
+    if (C.L->constraint) which = C.ntors > 0 ? 2 + int(b.x % uint32_t(C.ntors)) : -1;
+    else which = int(b.x % uint32_t(2 + C.ntors));
     Move mv;
-    if (which == 0) {
+    if (which < 0) {
+        mv.kind = 2;
+        mv.mag = 0.0f;
+        mv.moved = 0.0f;
+    } else if (which == 0) {
         const float3 a = rand_in_ball(rng);
         const float d0 = uni(fminf(fmaxf(2.0f * a.x + P.c0, S.box_lo[0]), S.box_hi[0]) - P.c0);
         const float d1 = uni(fminf(fmaxf(2.0f * a.y + P.c1, S.box_lo[1]), S.box_hi[1]) - P.c1);
@@ -826,11 +882,13 @@ __device__ Move mutate(const Ctx<APL>& C, WPose<APL>& P, Rng& rng) {
     return mv;
 }
 
-// randomize_pose_tile (mc.cu:18-104), non-constraint branch: centre drawn in
-// the box, super-Fibonacci orientation by chain index, each torsion drawn in a
-// random allowed range; applied through apply_update with alpha = 1.
+// randomize_pose_tile (mc.cu:18-104): centre drawn in the box, super-Fibonacci
+// orientation by chain index, each torsion drawn in a random allowed range;
+// applied through apply_update with alpha = 1. Constraint docking keeps centre
+// and orientation (:27-35), the align bias keeps the centre (:36-39); the centre
+// block is drawn either way.
 template <int APL>
-__device__ void randomize(const Ctx<APL>& C, WPose<APL>& P, Rng& rng, int idx, int n) {
+__device__ __forceinline__ void randomize(const Ctx<APL>& C, WPose<APL>& P, Rng& rng, int idx, int n) {
     const SceneDev& S = *C.S;
     const int ln = lane();
     const uint4 b = rng.next();
@@ -860,12 +918,13 @@ __device__ void randomize(const Ctx<APL>& C, WPose<APL>& P, Rng& rng, int idx, i
     // direction vector: lanes 0..5 rigid part, lane 6 + t torsion t
     const float sh = __shfl(tv, (ln - 6) & 63);
     float d = (ln >= 6 && ln < 6 + C.ntors) ? sh : 0.0f;
-    d = ln == 0 ? gc0 : d;
-    d = ln == 1 ? gc1 : d;
-    d = ln == 2 ? gc2 : d;
-    d = ln == 3 ? rv.x : d;
-    d = ln == 4 ? rv.y : d;
-    d = ln == 5 ? rv.z : d;
+    const bool keep_c = C.L->constraint || C.L->bias_type == 2, keep_o = C.L->constraint;
+    d = ln == 0 ? (keep_c ? 0.0f : gc0) : d;
+    d = ln == 1 ? (keep_c ? 0.0f : gc1) : d;
+    d = ln == 2 ? (keep_c ? 0.0f : gc2) : d;
+    d = ln == 3 ? (keep_o ? 0.0f : rv.x) : d;
+    d = ln == 4 ? (keep_o ? 0.0f : rv.y) : d;
+    d = ln == 5 ? (keep_o ? 0.0f : rv.z) : d;
     apply_update(C, P, d, 1.0f);
 }
 

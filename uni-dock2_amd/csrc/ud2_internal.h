@@ -33,6 +33,14 @@ struct LigSetDev {
     const float2* ranges;                  // (lo, hi) radians
     const unsigned long long* tors_mask;   // [ntors_total][kMaxApl] rotated-atom bit masks
     const uint2* pairs;                    // {i | j << 8 | hyd << 16 | hbond << 17, bits of R_i + R_j}
+    // Bias sites (FlexParamVina::inds_bias / params_bias, E/src/cuda/task.cu:157-168)
+    // and the task options that change the search (DockParam, E/src/model/model.h:151-168).
+    const int32_t* bias_start;             // [atoms + 1] CSR over the set's atoms; null: no sites
+    const float4* bias_site;               // x, y, z, V (well depth, or affinity for align)
+    const float* bias_r2;                  // Gaussian width r2 of each site (pos)
+    int32_t bias_type;                     // 0 no, 1 pos, 2 align (BiasType, model.h:145-149)
+    float bias_k;                          // scale of V
+    int32_t constraint;                    // constraint docking: no rigid degrees of freedom
 };
 
 // Device view of a scene: receptor, box and (optional) grid maps.
@@ -128,7 +136,9 @@ struct ScoreArgs {
     int32_t nchain;
     const int32_t* sel;
     int32_t n_sel;
-    float* out3;  // [n_sel][3]: intra, inter, penalty
+    float* out4;          // [n_sel][4]: intra, inter, penalty, bias (may be null)
+    float* decomp;        // [n_sel][decomp_stride][5] per-atom inter terms (decomposition launch)
+    int32_t decomp_stride;
 };
 
 struct GridArgs {
@@ -221,7 +231,7 @@ hipError_t launch_mc(McArgs a, int apl, int n_blocks, hipStream_t s);
 hipError_t launch_opt(OptArgs a, int apl, hipStream_t s);
 hipError_t launch_posevec(PoseVecArgs a, int apl, int n_blocks, hipStream_t s);
 hipError_t launch_cluster(const ClusterArgs& a, hipStream_t s);
-hipError_t launch_final_score(const ScoreArgs& a, int apl, hipStream_t s);
+hipError_t launch_final_score(const ScoreArgs& a, int apl, bool decomp, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 int mc_blocks_per_cu(int apl, int lds_bytes_per_block);
 // LDS floats of a wave slice before its Hessian tile.

@@ -266,9 +266,24 @@ __global__ __launch_bounds__(256) void cluster_kernel(ClusterArgs A) {
     if (threadIdx.x == 0) A.nleaders[lig] = nlead;
 }
 
+// Weighted Vina terms of one heavy pair at surface distance d (Vina::eval_decomp,
+// E/src/score/vina.h:256-274): gauss1, gauss2, repulsion, hydrophobic, hbond,
+// with the expressions of eval_ef.
+__device__ __forceinline__ void eval_terms(float d, int flags, float t[5]) {
+    t[0] = -0.035579f * __expf(-4.0f * d * d);
+    const float t2 = (d - 3.0f) * 0.5f;
+    t[1] = -0.005156f * __expf(-t2 * t2);
+    const float dn = fminf(d, 0.0f);
+    t[2] = 0.840245f * dn * dn;
+    t[3] = (flags & 1) ? -0.035069f * fminf(fmaxf(1.5f - d, 0.0f), 1.0f) : 0.0f;
+    t[4] = (flags & 2) ? -0.587439f * fminf(fmaxf(-d * (1.0f / 0.7f), 0.0f), 1.0f) : 0.0f;
+}
+
 // Final re-score of selected poses, score() (E/src/score/score.cpp:12-108):
-// no curl, intra pairs once, direct inter sum, penalty at the fixed slope.
-template <int APL>
+// no curl, intra pairs once, direct inter sum, penalty at the fixed slope, bias
+// sites. DECOMP: instead, the per-atom split of the inter sum into its five
+// terms, score_decomp() (score.cpp:115-147).
+template <int APL, bool DECOMP>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void final_score_kernel(ScoreArgs A) {
     const int w = threadIdx.x / kWave;
     const int s = blockIdx.x * kWavesPerBlock + w;
@@ -278,34 +293,59 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void final_score_kernel(Scor
     const DevLig d = A.L.ligs[lig];
     const float* xb = A.x + ((size_t)d.atom_off * A.nchain + (size_t)chain * d.natom) * 3;
     const SceneDev& S = A.S;
+    const LigSetDev& L = A.L;
     const int ln = lane();
-    float e_inter = 0.0f, e_pen = 0.0f, e_intra = 0.0f;
+    float e_inter = 0.0f, e_pen = 0.0f, e_intra = 0.0f, e_bias = 0.0f;
 #pragma unroll
     for (int k = 0; k < APL; k++) {
         const int a = ln + kWave * k;
         if (a >= d.natom) continue;
-        const int ti = __float_as_int(A.L.atom_x0[d.atom_off + a].w);
-        if (ti == kTypeH) continue;
+        const int ti = __float_as_int(L.atom_x0[d.atom_off + a].w);
         const float px = xb[3 * a], py = xb[3 * a + 1], pz = xb[3 * a + 2];
-        const float ri = c_vdw[ti];
-        for (int j = 0; j < S.nrec; j++) {
-            const float4 r = S.rec[j];
-            float rx = r.x - px, ry = r.y - py, rz = r.z - pz;
-            float rr = rx * rx + ry * ry + rz * rz;
-            if (rr < kR2Cut) {
-                const int tj = __float_as_int(r.w);
-                float dd;
-                e_inter += eval_ef(sqrtf(rr) - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
+        if (!DECOMP && L.bias_type != 0 && L.bias_start) {
+            const int j1 = L.bias_start[d.atom_off + a + 1];
+            for (int j = L.bias_start[d.atom_off + a]; j < j1; j++) {
+                const float4 b = L.bias_site[j];
+                float sdummy;
+                e_bias += bias_site_energy(L.bias_type, L.bias_k, b, L.bias_r2[j], ti, b.x - px, b.y - py, b.z - pz,
+                                           sdummy);
             }
         }
-        if (px < S.box_lo[0]) e_pen += S.box_lo[0] - px;
-        if (px > S.box_hi[0]) e_pen += px - S.box_hi[0];
-        if (py < S.box_lo[1]) e_pen += S.box_lo[1] - py;
-        if (py > S.box_hi[1]) e_pen += py - S.box_hi[1];
-        if (pz < S.box_lo[2]) e_pen += S.box_lo[2] - pz;
-        if (pz > S.box_hi[2]) e_pen += pz - S.box_hi[2];
+        float t5[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (ti != kTypeH) {
+            const float ri = c_vdw[ti];
+            for (int j = 0; j < S.nrec; j++) {
+                const float4 r = S.rec[j];
+                float rx = r.x - px, ry = r.y - py, rz = r.z - pz;
+                float rr = rx * rx + ry * ry + rz * rz;
+                if (rr < kR2Cut) {
+                    const int tj = __float_as_int(r.w);
+                    if (DECOMP) {
+                        float t[5];
+                        eval_terms(sqrtf(rr) - (ri + c_vdw[tj]), pair_flags(ti, tj), t);
+#pragma unroll
+                        for (int q = 0; q < 5; q++) t5[q] += t[q];
+                    } else {
+                        float dd;
+                        e_inter += eval_ef(sqrtf(rr) - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
+                    }
+                }
+            }
+            if (px < S.box_lo[0]) e_pen += S.box_lo[0] - px;
+            if (px > S.box_hi[0]) e_pen += px - S.box_hi[0];
+            if (py < S.box_lo[1]) e_pen += S.box_lo[1] - py;
+            if (py > S.box_hi[1]) e_pen += py - S.box_hi[1];
+            if (pz < S.box_lo[2]) e_pen += S.box_lo[2] - pz;
+            if (pz > S.box_hi[2]) e_pen += pz - S.box_hi[2];
+        }
+        if (DECOMP) {
+            float* o = A.decomp + ((size_t)s * A.decomp_stride + a) * 5;
+#pragma unroll
+            for (int q = 0; q < 5; q++) o[q] = t5[q];
+        }
     }
-    const uint2* pr = A.L.pairs + d.pair_off;
+    if (DECOMP) return;
+    const uint2* pr = L.pairs + d.pair_off;
     for (int p = ln; p < d.npair; p += kWave) {
         const uint2 rec = pr[p];
         const int i = rec.x & 255, j = (rec.x >> 8) & 255, fl = (rec.x >> 16) & 3;
@@ -319,10 +359,12 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void final_score_kernel(Scor
     e_inter = wave_sum(e_inter);
     e_pen = wave_sum(e_pen) * kSlope;
     e_intra = wave_sum(e_intra);
-    if (ln == 0) {
-        A.out3[(size_t)s * 3] = e_intra;
-        A.out3[(size_t)s * 3 + 1] = e_inter;
-        A.out3[(size_t)s * 3 + 2] = e_pen;
+    e_bias = wave_sum(e_bias);
+    if (ln == 0 && A.out4) {
+        A.out4[(size_t)s * 4] = e_intra;
+        A.out4[(size_t)s * 4 + 1] = e_inter;
+        A.out4[(size_t)s * 4 + 2] = e_pen;
+        A.out4[(size_t)s * 4 + 3] = e_bias;
     }
 }
 
@@ -406,19 +448,20 @@ hipError_t launch_cluster(const ClusterArgs& a, hipStream_t s) {
 }
 
 template <int APL>
-static hipError_t launch_score_t(const ScoreArgs& a, hipStream_t s) {
+static hipError_t launch_score_t(const ScoreArgs& a, bool decomp, hipStream_t s) {
     int nb = (a.n_sel + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(final_score_kernel<APL>, dim3(nb), dim3(kWave * kWavesPerBlock), 0, s, a);
+    if (decomp) hipLaunchKernelGGL((final_score_kernel<APL, true>), dim3(nb), dim3(kWave * kWavesPerBlock), 0, s, a);
+    else hipLaunchKernelGGL((final_score_kernel<APL, false>), dim3(nb), dim3(kWave * kWavesPerBlock), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_final_score(const ScoreArgs& a, int apl, hipStream_t s) {
+hipError_t launch_final_score(const ScoreArgs& a, int apl, bool decomp, hipStream_t s) {
     if (a.n_sel <= 0) return hipSuccess;
     switch (apl) {
-        case 1: return launch_score_t<1>(a, s);
-        case 2: return launch_score_t<2>(a, s);
-        case 3: return launch_score_t<3>(a, s);
-        case 4: return launch_score_t<4>(a, s);
+        case 1: return launch_score_t<1>(a, decomp, s);
+        case 2: return launch_score_t<2>(a, decomp, s);
+        case 3: return launch_score_t<3>(a, decomp, s);
+        case 4: return launch_score_t<4>(a, decomp, s);
         default: return hipErrorInvalidValue;
     }
 }

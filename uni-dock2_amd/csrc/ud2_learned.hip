@@ -35,7 +35,7 @@ static __constant__ int c_fclass[kNumTypes] = {-1, 1, 0, 1, 2, 3, 4, 3, 5, 6, 5,
 // globals in xrow[64..79]. The per-atom rows go through the LDS scratch at
 // `sc` (8 floats per atom) and lane j sums feature j.
 template <int APL>
-__device__ void pose_features(const Ctx<APL>& C, const WPose<APL>& P, const Move& mv, float e_acc, int sc,
+__device__ __forceinline__ void pose_features(const Ctx<APL>& C, const WPose<APL>& P, const Move& mv, float e_acc, int sc,
                               uint16_t* xrow) {
     const SceneDev& S = *C.S;
     const int ln = lane();
@@ -96,9 +96,12 @@ __device__ void pose_features(const Ctx<APL>& C, const WPose<APL>& P, const Move
     wave_lds_fence();
 }
 
-// Persistent learned-proposal MC kernel.
+// Persistent learned-proposal MC kernel. One-atom-per-lane ligands fit three
+// waves per SIMD in LDS; the occupancy hint holds the register allocation to
+// the 168 VGPRs that allows.
 template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void lmc_kernel(LmcArgs A) {
+__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(APL == 1 ? 3 : 1)))
+void lmc_kernel(LmcArgs A) {
     const int w = threadIdx.x / kWave;
     const int base = w * A.lds_floats_per_wave;
     const int nchain = A.P.nchain;

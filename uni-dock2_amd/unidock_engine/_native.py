@@ -86,6 +86,8 @@ def lib():
         "ud2_scene_destroy": (C.c_int, [vp]),
         "ud2_ligset_create": (C.c_int, [vp, vp]),
         "ud2_ligset_destroy": (C.c_int, [vp]),
+        "ud2_ligset_set_bias": (C.c_int, [vp, i32, f32, vp, vp]),
+        "ud2_ligset_set_constraint": (C.c_int, [vp, i32]),
         "ud2_ligset_pose_layout": (C.c_int, [vp, i32, vp, vp, vp]),
         "ud2_init_poses": (C.c_int, [vp, vp, vp]),
         "ud2_score_posevecs": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f32, f32, vp]),
@@ -96,6 +98,7 @@ def lib():
         "ud2_mc": (C.c_int, [vp, vp, vp, vp, vp]),
         "ud2_cluster": (C.c_int, [vp, vp, f32, vp, vp, vp]),
         "ud2_final_score": (C.c_int, [vp, vp, vp, vp, i32, vp, vp]),
+        "ud2_score_decomp": (C.c_int, [vp, vp, vp, vp, i32, i32, vp, vp]),
         "ud2_mlp_create": (C.c_int, [vp, vp]),
         "ud2_mlp_set_params": (C.c_int, [vp, vp]),
         "ud2_mlp_get_params": (C.c_int, [vp, vp]),
@@ -117,9 +120,10 @@ def lib():
 EXPORTED = (
     "ud2_version", "ud2_last_error", "ud2_device_count", "ud2_set_device", "ud2_synchronize", "ud2_scene_create",
     "ud2_scene_build_grids", "ud2_scene_set_grids", "ud2_scene_grid_info", "ud2_scene_download_maps",
-    "ud2_scene_destroy", "ud2_ligset_create", "ud2_ligset_destroy", "ud2_ligset_pose_layout", "ud2_init_poses",
+    "ud2_scene_destroy", "ud2_ligset_create", "ud2_ligset_destroy", "ud2_ligset_set_bias",
+    "ud2_ligset_set_constraint", "ud2_ligset_pose_layout", "ud2_init_poses",
     "ud2_score_posevecs", "ud2_energy_grad", "ud2_pose_update", "ud2_line_search", "ud2_optimize", "ud2_mc",
-    "ud2_cluster", "ud2_final_score", "ud2_mlp_create", "ud2_mlp_set_params", "ud2_mlp_get_params",
+    "ud2_cluster", "ud2_final_score", "ud2_score_decomp", "ud2_mlp_create", "ud2_mlp_set_params", "ud2_mlp_get_params",
     "ud2_mlp_destroy", "ud2_mlp_forward", "ud2_mlp_backward", "ud2_mlp_train_step", "ud2_features", "ud2_lmc",
 )
 
@@ -222,10 +226,18 @@ def pack_ligands(ligands) -> dict:
     )
 
 
-class LigSet:
-    """A batch of ligand topologies on the device."""
+BIAS_TYPES = {"no": 0, "pos": 1, "align": 2}  # UD2_BIAS_NO / POS / ALIGN
 
-    def __init__(self, ligands):
+
+class LigSet:
+    """A batch of ligand topologies on the device.
+
+    ``bias`` ("no", "pos", "align") and ``bias_k`` select the bias term over the
+    ligands' bias sites (``Ligand.bias_start`` / ``Ligand.bias``);
+    ``constraint`` turns on constraint docking (no rigid degrees of freedom).
+    """
+
+    def __init__(self, ligands, bias: str = "no", bias_k: float = 0.1, constraint: bool = False):
         self.ligands = list(ligands)
         self.arrays = pack_ligands(self.ligands)
         a = {k: (v if v.size else np.zeros(1, v.dtype)) for k, v in self.arrays.items()}
@@ -235,6 +247,15 @@ class LigSet:
             "range_off", "ranges", "pairs")])
         self.h = C.c_void_p()
         check(lib().ud2_ligset_create(C.byref(b), C.byref(self.h)), "ud2_ligset_create")
+        if bias not in BIAS_TYPES:
+            raise ValueError(f"Not supported bias: {bias} doesn't belong to (no, pos, align)")
+        if BIAS_TYPES[bias] != 0:
+            off, sites = bias_arrays(self.ligands)
+            self._bias = (off, sites)
+            check(lib().ud2_ligset_set_bias(self.h, BIAS_TYPES[bias], float(bias_k), _ptr(off), _ptr(sites)),
+                  "ud2_ligset_set_bias")
+        if constraint:
+            check(lib().ud2_ligset_set_constraint(self.h, 1), "ud2_ligset_set_constraint")
 
     @property
     def n_lig(self) -> int:
@@ -263,6 +284,24 @@ class LigSet:
             self.close()
         except Exception:
             pass
+
+
+def bias_arrays(ligands):
+    """CSR of bias sites over the atoms of a ligand batch: offsets [atoms+1] (int32)
+    and sites [n, 5] (float32: x, y, z, V, r2)."""
+    counts, sites = [], []
+    for lig in ligands:
+        bs = getattr(lig, "bias_start", None)
+        if bs is None:
+            counts.append(np.zeros(lig.natom, np.int64))
+            continue
+        counts.append(np.diff(np.asarray(bs, np.int64)))
+        if np.size(lig.bias):
+            sites.append(np.asarray(lig.bias, np.float32).reshape(-1, 5))
+    off = np.zeros(sum(len(c) for c in counts) + 1, np.int32)
+    off[1:] = np.cumsum(np.concatenate(counts))
+    sites = np.concatenate(sites) if sites else np.zeros((1, 5), np.float32)
+    return off, np.ascontiguousarray(sites, np.float32)
 
 
 def stream_ptr(stream=None):

@@ -104,7 +104,8 @@ def fmt_float(v: float) -> str:
 
 
 def dumps_poses(results) -> str:
-    """write_poses_to_json, src/format/json.cpp:88-191 (compact, no whitespace)."""
+    """write_poses_to_json, src/format/json.cpp:88-191 (compact, no whitespace); a pose
+    with a per-atom energy decomposition gets its "decomp" member (json.cpp:134-147)."""
     parts = []
     for r in results:
         poses = []
@@ -112,7 +113,11 @@ def dumps_poses(results) -> str:
             e = ",".join(fmt_float(x) for x in r.energies[k])
             c = ",".join(fmt_float(x) for x in np.asarray(r.coords[k], np.float32))
             d = ",".join(fmt_float(x) for x in np.asarray(r.dihedrals[k], np.float32))
-            poses.append('{"energy":[' + e + '],"coords":[' + c + '],"dihedrals":[' + d + "]}")
+            extra = ""
+            if k < len(getattr(r, "decomp", ())):
+                extra = ',"decomp":[' + ",".join(
+                    "[" + ",".join(fmt_float(x) for x in row) + "]" for row in np.asarray(r.decomp[k], np.float32)) + "]"
+            poses.append('{"energy":[' + e + '],"coords":[' + c + '],"dihedrals":[' + d + "]" + extra + "}")
         parts.append(json.dumps(r.name) + ":[" + ",".join(poses) + "]")
     return "{" + ",".join(parts) + "}"
 
@@ -183,14 +188,13 @@ class DockingPipeline:
         p = dict(self.params)
         # the reference serialises the cached objects to JSON text and parses them back
         doc = json.loads(json.dumps(self.json_data))
-        if p["bias"] not in ("no", "pos", "align"):
+        if p["bias"] not in ("no", "pos", "align"):  # apply_bias, core.cpp:86-97
             raise ValueError(f"Not supported bias: {p['bias']} doesn't belong to (no, pos, align)")
-        if p["bias"] != "no" or p["constraint_docking"] or p["energy_decomp"]:
-            raise NotImplementedError("bias terms, constraint docking and energy decomposition are not "
-                                      "implemented by this engine yet")
         apply_search_mode(p)
         if p["opt_steps"] < 0:
             p["opt_steps"] = -1
+        if p["constraint_docking"]:  # prepare_context_by_input, core.cpp:142-146
+            p["randomize"] = False
         apply_task(p)
         os.makedirs(p["output_dir"], exist_ok=True)
         N.check(N.lib().ud2_set_device(p["gpu_device_id"]), "ud2_set_device")
@@ -214,9 +218,9 @@ class DockingPipeline:
                 prm = DockParams(nchain=p["exhaustiveness"], mc_steps=p["mc_steps"], opt_steps=opt_steps,
                                  refine_steps=p["refine_steps"], num_pose=p["num_pose"],
                                  rmsd_limit=p["rmsd_limit"], energy_range=p["energy_range"], seed=p["seed"],
-                                 randomize=p["randomize"], mode=self.inter_mode)
+                                 randomize=p["randomize"], mode=self.inter_mode, energy_decomp=p["energy_decomp"])
                 t0 = time.time()
-                ls = N.LigSet(batch)
+                ls = N.LigSet(batch, bias=p["bias"], bias_k=p["bias_k"], constraint=p["constraint_docking"])
                 results, _ = eng.dock(ls, prm)
                 path = os.path.join(p["output_dir"], f"{p['name_json']}_{batch_id}.json")
                 write_json_with_retry(path, dumps_poses(results))

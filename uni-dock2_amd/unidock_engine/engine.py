@@ -5,8 +5,10 @@ One call of :meth:`DockEngine.dock` is the reference's DockTask::run
 Monte Carlo search (ud2_mc), clustering (ud2_cluster), refinement with the
 penalty-slope ramp (ud2_optimize, DockTask.cpp:155-186), filtering by
 energy_range / num_pose (DockTask.cpp:188-220) and final scoring with the
-affinity of DockTask.cpp:224-282 (ud2_final_score). Everything but the filter
-runs on the device; the filter is a sort of a few floats per ligand.
+affinity of DockTask.cpp:224-282 (ud2_final_score, and ud2_score_decomp for the
+energy decomposition). Bias terms and constraint docking are options of the
+ligand set (N.LigSet). Everything but the filter runs on the device; the filter
+is a sort of a few floats per ligand.
 """
 from __future__ import annotations
 
@@ -56,6 +58,7 @@ class DockParams:
     top_k: int = 4               # candidates the learned policy rescores exactly
     sel_temp: float = 0.0        # learned: > 0 draws one candidate with p ~ exp(-score / sel_temp) instead
     n_start: int = 1             # > 1: randomised starts per chain, the best kept
+    energy_decomp: bool = False  # per-atom inter terms of every output pose (score_decomp)
 
 
 @dataclass
@@ -65,6 +68,7 @@ class LigandResult:
     coords: list = field(default_factory=list)      # [n_out] arrays (natom*3,)
     dihedrals: list = field(default_factory=list)   # [n_out] arrays (ntors,) degrees
     energies: list = field(default_factory=list)    # [n_out] [affinity, total, intra, inter, penalty, conf_indep, bias]
+    decomp: list = field(default_factory=list)      # [n_out] arrays (natom, 5), with DockParams.energy_decomp
 
 
 class DockEngine:
@@ -166,23 +170,30 @@ class DockEngine:
                 if cnt >= prm.num_pose:
                     break
         keep = torch.as_tensor(np.asarray(keep_ids, np.int32), device="cuda")
-        out3 = torch.zeros((max(len(keep_ids), 1), 3), device="cuda")
+        out4 = torch.zeros((max(len(keep_ids), 1), 4), device="cuda")
         if keep_ids:
             N.check(lib.ud2_final_score(self.scene.h, ligset.h, pb.ref, C.c_void_p(keep.data_ptr()), len(keep_ids),
-                                        C.c_void_p(out3.data_ptr()), st), "ud2_final_score")
+                                        C.c_void_p(out4.data_ptr()), st), "ud2_final_score")
+        stride = max(l.natom for l in ligs)
+        decomp = None
+        if prm.energy_decomp and keep_ids:
+            decomp = torch.zeros((len(keep_ids), stride, 5), device="cuda")
+            N.check(lib.ud2_score_decomp(self.scene.h, ligset.h, pb.ref, C.c_void_p(keep.data_ptr()), len(keep_ids),
+                                         stride, C.c_void_p(decomp.data_ptr()), st), "ud2_score_decomp")
         if sync:
             torch.cuda.synchronize()
         results = [LigandResult(l.name) for l in ligs]
         if not sync:
             return results, pb
-        sc = out3[:len(keep_ids)].cpu().numpy()
+        sc = out4[:len(keep_ids)].cpu().numpy()
+        dc = decomp.cpu().numpy() if decomp is not None else None
         x_h = pb.x.cpu().numpy()
         d_h = pb.dihe.cpu().numpy()
         intra_r1 = {}
         for k, (pid, l) in enumerate(zip(keep_ids, owner)):
             L = ligs[l]
             ch = pid - l * nchain
-            intra, inter, pen = (np.float32(v) for v in sc[k])
+            intra, inter, pen, bias = (np.float32(v) for v in sc[k])
             if l not in intra_r1:
                 intra_r1[l] = intra
             total = np.float32(intra + inter + pen)
@@ -194,5 +205,7 @@ class DockEngine:
             r.pose_ids.append(pid)
             r.coords.append(x_h[xo:xo + 3 * L.natom].copy())
             r.dihedrals.append((d_h[do:do + L.ntors] * RAD2DEG_F32).astype(np.float32))
-            r.energies.append(np.asarray([aff, total, intra, inter, pen, np.float32(aff - e_inter), 0.0], np.float32))
+            r.energies.append(np.asarray([aff, total, intra, inter, pen, np.float32(aff - e_inter), bias], np.float32))
+            if dc is not None:
+                r.decomp.append(dc[k, :L.natom].copy())
         return results, pb

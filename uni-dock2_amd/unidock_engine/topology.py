@@ -86,6 +86,9 @@ class Ligand:
     pairs: np.ndarray        # (P, 2) int32
     ff_types: np.ndarray | None = None
     charges: np.ndarray | None = None
+    # bias sites: atom a owns rows bias_start[a] .. bias_start[a+1]-1 of bias (x, y, z, V, r2)
+    bias_start: np.ndarray | None = None  # (n+1,) int32, or None: no sites
+    bias: np.ndarray | None = None        # (B, 5) float32
     json_obj: dict | None = field(default=None, repr=False)
     pairs_1213: set | None = field(default=None, repr=False)
     pairs_14: set | None = field(default=None, repr=False)
@@ -227,7 +230,26 @@ def ligand_from_json(name: str, obj: dict, use_tor_lib: bool = False) -> Ligand:
     pairs = intra_pairs(n, p1213, p14, frags)
     lig = make_ligand(name, coords, types, dihe, axis, rot_lists, range_lists, pairs, ff, q, obj)
     lig.pairs_1213, lig.pairs_14, lig.frags = p1213, p14, frags
+    lig.bias_start, lig.bias = bias_sites(atoms)
     return lig
+
+
+def bias_sites(atoms: list):
+    """Per-atom bias sites, the ninth field of an atom line (rapidjson_parser.cpp:113-124):
+    a list of [x, y, z, V, r2] per site, in atom order (as task.cu:157-168 lays them out).
+    Returns (bias_start, bias) or (None, None) when no atom has a site."""
+    start = np.zeros(len(atoms) + 1, dtype=np.int32)
+    rows = []
+    for ia, a in enumerate(atoms):
+        sites = a[8] if len(a) > 8 else []
+        for b in sites:
+            if len(b) < 5:
+                raise ValueError(f"bias site of atom {ia} has {len(b)} values, expected 5")
+            rows.append([float(v) for v in b[:5]])
+        start[ia + 1] = len(rows)
+    if not rows:
+        return None, None
+    return start, np.asarray(rows, dtype=np.float32).reshape(-1, 5)
 
 
 def make_ligand(name, coords, types, dihedrals, axis, rot_lists, range_lists, pairs, ff_types=None, charges=None,
