@@ -1,0 +1,111 @@
+"""The large uneven configuration (4,096 ligands with 10-80 heavy atoms, log-uniform, and 0-20
+rotatable bonds; 256 chains x 40 MC steps with the learned proposal; greedy clustering over 512
+candidate poses per ligand) exercised at its real shapes: ligands past 64 atoms run the
+two-atoms-per-lane kernels, and cluster membership over 512 poses must be bit-exact against the
+oracle on identical pose sets."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+VP = C.c_void_p
+PROBES = [2, 3, 4, 5, 6, 10, 11, 15]
+
+
+def dptr(t):
+    return VP(t.data_ptr())
+
+
+@pytest.fixture(scope="module")
+def c4_ligs():
+    """Six c4 ligands: the smallest, the largest (79 atoms) and four more over 64 atoms or with many torsions."""
+    from unidock_engine import synthetic as S
+    ligs = [s.ligand for s in S.config4_ligands(64)]
+    order = np.argsort([l.natom for l in ligs])
+    pick = [int(order[0]), int(order[-1]), int(order[-2]), int(order[-3])]
+    pick += [int(i) for i in np.argsort([-l.ntors for l in ligs]) if int(i) not in pick][:2]
+    return [ligs[i] for i in pick]
+
+
+@pytest.fixture(scope="module")
+def scene(gpu, receptor, box):
+    from unidock_engine import synthetic as S
+    sc = gpu.Scene(receptor, box)
+    sc.build_grids(S.GRID_SPACING, PROBES)
+    return sc
+
+
+def test_cluster_bit_exact_at_512_poses(gpu, scene, c4_ligs):
+    """512 poses per ligand from the engine's own search (randomised starts, 2 MC steps x BFGS 4),
+    clustered on the GPU at 1.0 A (the configuration's limit), 2.0 A and 4.0 A; leaders in selection
+    order equal the oracle's greedy clustering of the same FP32 poses and energies."""
+    import torch
+    print(f"c4 clustering, natoms {[l.natom for l in c4_ligs]}", flush=True)
+    assert max(l.natom for l in c4_ligs) > 64
+    ls = gpu.LigSet(c4_ligs)
+    nchain = 512
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    mc = gpu.McParams(mc_steps=2, opt_steps=4, randomize=1, seed=41, mode=gpu.MODE_GRID, lig_index_base=0,
+                      stats_d=None)
+    gpu.check(gpu.lib().ud2_mc(scene.h, ls.h, pb.ref, C.byref(mc), gpu.stream_ptr()), "ud2_mc")
+    torch.cuda.synchronize()
+    n_lig = len(c4_ligs)
+    leaders = torch.zeros((n_lig, nchain), dtype=torch.int32, device="cuda")
+    nlead = torch.zeros(n_lig, dtype=torch.int32, device="cuda")
+    es = pb.e[:n_lig * nchain].cpu().numpy().reshape(n_lig, nchain)
+    assert np.isfinite(es).all()
+    merged = 0
+    for limit in (1.0, 2.0, 4.0):
+        gpu.check(gpu.lib().ud2_cluster(ls.h, pb.ref, limit, dptr(leaders), dptr(nlead), gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        for l, lig in enumerate(c4_ligs):
+            xs = np.stack([pb.pose(l, c)["x"] for c in range(nchain)]).astype(np.float32)
+            ref = O.cluster(es[l], xs, lig.types, limit)
+            got = leaders[l, :int(nlead[l])].cpu().numpy()
+            merged += len(ref) < nchain
+            assert np.array_equal(got, ref), (limit, l, len(got), len(ref))
+    assert merged >= 6  # the limits do merge poses, so membership is really tested
+
+
+def test_learned_search_at_c4_shapes(gpu, scene, c4_ligs):
+    """The learned kernel (packaged network, 4 randomised starts, 16 candidates, sampling at T = 1)
+    on the c4 shapes with 256 chains: every stored pose is a real conformer of its ligand and its
+    stored energy is the exact search energy of that pose (V_CAP 1000), re-evaluated separately.
+    Bond lengths within 1 %: the kinematics update absolute FP32 coordinates, as the reference's do
+    (E/src/cuda/mc.cu:224 notes the low precision of that rotation); the FP32 oracle drifts by 2.1e-3 on the 79-atom
+    ligand in 16 chains of this run's length, its FP64 flavour by 1e-12."""
+    import torch
+    from unidock_engine import proposal as LP
+    ls = gpu.LigSet(c4_ligs)
+    nchain = 256
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    mlp = gpu.Mlp(LP.load())
+    lp = gpu.LmcParams()
+    lp.mc = gpu.McParams(mc_steps=4, opt_steps=8, randomize=1, seed=43, mode=gpu.MODE_GRID, lig_index_base=0,
+                         stats_d=None)
+    lp.policy, lp.n_cand, lp.top_k, lp.sel_temp, lp.n_start = gpu.POLICY_LEARNED, 16, 1, 1.0, 4
+    gpu.check(gpu.lib().ud2_lmc(scene.h, ls.h, pb.ref, C.byref(lp), mlp.h, gpu.stream_ptr()), "ud2_lmc")
+    torch.cuda.synchronize()
+    n = len(c4_ligs) * nchain
+    e = pb.e[:n].cpu().numpy()
+    assert np.isfinite(e).all()  # chains that end outside the box carry the 1e6 penalty slope
+    sel = torch.arange(n, dtype=torch.int32, device="cuda")
+    e2 = torch.zeros(n, device="cuda")
+    g = torch.zeros((n, 64), device="cuda")
+    gpu.check(gpu.lib().ud2_energy_grad(scene.h, ls.h, pb.ref, dptr(sel), n, gpu.MODE_GRID, 1000.0, 1e6, dptr(e2),
+                                        dptr(g), gpu.stream_ptr()))
+    torch.cuda.synchronize()
+    e2 = e2.cpu().numpy()
+    assert np.all(np.abs(e2 - e) <= 1e-4 * np.maximum(1.0, np.abs(e))), np.abs(e2 - e).max()
+    for l, lig in enumerate(c4_ligs):
+        x0 = lig.coords
+        for c in range(0, nchain, 37):
+            x = pb.pose(l, c)["x"].reshape(-1, 3)
+            for a, b in lig.axis:
+                d0 = np.linalg.norm(x0[a] - x0[b])
+                assert abs(np.linalg.norm(x[a] - x[b]) - d0) < 1e-2 * d0

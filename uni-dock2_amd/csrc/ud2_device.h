@@ -149,6 +149,10 @@ __device__ __forceinline__ float4 rotvec_to_quat(float v0, float v1, float v2) {
     const float th = fsqrt(v0 * v0 + v1 * v1 + v2 * v2);
     float s, c;
     __sincosf(th * 0.5f, &s, &c);  // th is a step-sized angle: the hardware sin/cos is accurate here
+    // renormalised: c^2 + s^2 = 1 to an ulp, so R(q) does not scale the ligand (|q|^2 would, per update)
+    const float rn = frsq(c * c + s * s);
+    s *= rn;
+    c *= rn;
     const float k = th > kEps ? s * frcp(th) : 0.0f;
     return th > kEps ? make_float4(c, v0 * k, v1 * k, v2 * k) : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -404,7 +408,8 @@ __device__ __forceinline__ void apply_torsion(const Ctx<APL>& C, WPose<APL>& P, 
     else { const float il = frcp(l); ux *= il; uy *= il; uz *= il; }
     float s, c;
     __sincosf(inc, &s, &c);
-    ux = uni(ux); uy = uni(uy); uz = uni(uz); s = uni(s); c = uni(c);
+    const float rn = frsq(c * c + s * s);  // unit (c, s): no growth of the rotated fragment per step
+    ux = uni(ux); uy = uni(uy); uz = uni(uz); s = uni(s * rn); c = uni(c * rn);
     const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
 #pragma unroll
     for (int k = 0; k < APL; k++) {
