@@ -12,12 +12,12 @@ scoring.
 
   plain    128 chains x 40 MC steps, one random mutation + BFGS of
            (25 + natom_max) / 3 iterations per step (the reference's search);
-  learned  the learned engine: 32 chains x 40 MC steps; each chain starts from
+  learned  the learned engine: 36 chains x 40 MC steps; each chain starts from
            the best of 4 randomised starts by the network, and each step draws
            16 mutations, scores them with the network on MFMA and samples one
            with p ~ exp(-score / 1 kcal/mol) for BFGS and the Metropolis test.
 
-The plain engine is also timed at the learned engine's budget (32 x 40,
+The plain engine is also timed at the learned engine's budget (36 x 40,
 ``config.plain_same_budget``): on this workload that control shows how much
 of the speedup comes from the smaller budget rather than from the network.
 
@@ -65,7 +65,7 @@ def parse():
                          "copy that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--search", choices=["learned", "plain"], default="learned",
                     help="learned: headline (learned vs plain); plain: the plain engine only")
-    ap.add_argument("--learned-nchain", type=int, default=32)
+    ap.add_argument("--learned-nchain", type=int, default=36)
     ap.add_argument("--learned-steps", type=int, default=40)
     ap.add_argument("--n-cand", type=int, default=16)
     ap.add_argument("--top-k", type=int, default=1)
@@ -198,12 +198,24 @@ def copy_probe():
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lmc_mc_kernels_c1_batch_traffic.md")
 
 
-DEFAULT_TRAFFIC_CONFIG = "1024 ligands grid; learned 32x40 starts 4 cand 16 T 1.0; plain 128x40"
+DEFAULT_TRAFFIC_CONFIG = "1024 ligands grid; learned 36x40 starts 4 cand 16 T 1.0; plain 128x40"
+
+
+def kernel_sources_digest() -> str:
+    """SHA-1 (12 hex digits) of the kernel sources: a traffic summary counts only for the build it measured."""
+    import hashlib
+    d = os.path.join(ROOT, "uni-dock2_amd", "csrc")
+    h = hashlib.sha1()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".h", ".hip", ".cpp")):
+            h.update(f.encode())
+            h.update(open(os.path.join(d, f), "rb").read())
+    return h.hexdigest()[:12]
 
 
 def profiled_traffic(key: str):
     """Bytes per launch measured with the FETCH_SIZE / WRITE_SIZE counters for the default
-    workload (profiles/), or None when that summary is for another configuration."""
+    workload (profiles/), or None when that summary is for another configuration or build."""
     vals = {}
     try:
         for line in open(TRAFFIC_SUMMARY):
@@ -212,7 +224,7 @@ def profiled_traffic(key: str):
                 vals[k] = v
     except OSError:
         return None
-    if vals.get("config") != DEFAULT_TRAFFIC_CONFIG or key not in vals:
+    if vals.get("config") != DEFAULT_TRAFFIC_CONFIG or vals.get("sources") != kernel_sources_digest() or key not in vals:
         return None
     return int(float(vals[key]))
 
@@ -379,7 +391,7 @@ def main():
         },
     }
     default_run = (args.n_lig, args.mode, args.nchain, args.mc_steps, args.learned_nchain, args.learned_steps,
-                   args.n_cand, args.sel_temp, args.n_start) == (1024, "grid", 128, 40, 32, 40, 16, 1.0, 4)
+                   args.n_cand, args.sel_temp, args.n_start) == (1024, "grid", 128, 40, 36, 40, 16, 1.0, 4)
     if default_run:
         t = profiled_traffic("lmc_traffic_bytes_per_launch") if learned else None
         if t:

@@ -240,7 +240,7 @@ int run_opt(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
         const int n = start[apl + 1] - start[apl];
         if (n == 0) continue;
         A.hdim_ld = odd_ld(dmax[apl]);
-        A.lds_floats_per_wave = lds_fixed_floats(apl) + dmax[apl] * A.hdim_ld;
+        A.lds_floats_per_wave = lds_round4(lds_head_floats(apl, A.hdim_ld) + dmax[apl] * A.hdim_ld);
         A.sel = ls->d_sel_all + start[apl];
         A.pos = ls->d_sel_all + n_sel + start[apl];
         A.n_sel = n;
@@ -497,6 +497,31 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
                 ranges.push_back(make_float2(lo, hi));
             }
         }
+        // Leaf-first application order: a torsion that rotates another's pivot is its ancestor,
+        // and deeper torsions go first. The order is valid when no torsion rotates the axis
+        // atoms of one applied after it (a proper torsion tree); a pose update can then take
+        // every pivot and axis from the coordinates before the update, as the result of the
+        // reference's index-order application (bfgs.cuh:565-567) is the same in exact arithmetic.
+        {
+            auto rotates = [&](int tp, int atom) {
+                return (mask[size_t(t0 + tp) * kMaxApl + atom / kWave] >> (atom % kWave)) & 1ull;
+            };
+            std::vector<int> depth(nt, 0), ord(nt);
+            for (int t = 0; t < nt; t++)
+                for (int tp = 0; tp < nt; tp++)
+                    if (tp != t && rotates(tp, axis[t0 + t].y)) depth[t]++;
+            std::iota(ord.begin(), ord.end(), 0);
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return depth[a] > depth[c]; });
+            bool ok = true;
+            for (int i = 0; i < nt && ok; i++)
+                for (int j = i + 1; j < nt; j++)
+                    if (rotates(ord[i], axis[t0 + ord[j]].x) || rotates(ord[i], axis[t0 + ord[j]].y)) {
+                        ok = false;
+                        break;
+                    }
+            for (int k = 0; k < nt; k++) axis[t0 + k].y |= ord[k] << 16;
+            d.leaf_first = ok ? 1 : 0;
+        }
         // Heavy pairs in diagonal order (j - i, i): 64 consecutive pairs then hit
         // distinct atoms on each side, so the kernel's LDS force atomics of one
         // instruction do not collide.
@@ -664,7 +689,8 @@ int ud2_score_posevecs(const ud2_scene* sc, const ud2_ligset* ls, int32_t lig, i
     A.v_cap = v_cap;
     A.slope = slope;
     A.mode = mode;
-    A.lds_floats_per_wave = lds_fixed_floats(d.apl);
+    A.hdim_ld = odd_ld(6 + d.ntors);
+    A.lds_floats_per_wave = lds_round4(lds_head_floats(d.apl, A.hdim_ld));
     const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
     const int64_t need = (n + kWavesPerBlock - 1) / kWavesPerBlock;
     const int64_t cap = int64_t(n_cus()) * posevec_blocks_per_cu(d.apl, lds_bytes);
@@ -757,7 +783,7 @@ int ud2_mc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const 
         A.n_order = cnt;
         A.counter = ls->d_counter + apl;
         A.hdim_ld = odd_ld(dmax);
-        A.lds_floats_per_wave = lds_fixed_floats(apl) + dmax * A.hdim_ld;
+        A.lds_floats_per_wave = lds_round4(lds_head_floats(apl, A.hdim_ld) + dmax * A.hdim_ld);
         const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
         if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
         const int64_t items = int64_t(cnt) * P->nchain;
@@ -1060,10 +1086,11 @@ int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, 
     std::vector<int32_t> h(n_sel);
     UD2_HIP(hipMemcpy(h.data(), sel_d, size_t(n_sel) * sizeof(int32_t), hipMemcpyDeviceToHost));
     const int64_t npose = int64_t(ls->n_lig) * P->nchain;
-    int apl = 1;
+    int apl = 1, dmax = 6;
     for (int32_t p : h) {
         if (p < 0 || p >= npose) return fail(UD2_ERR_ARG, "pose id out of range: " + std::to_string(p));
         apl = std::max(apl, int(ls->h_ligs[p / P->nchain].apl));
+        dmax = std::max(dmax, 6 + int(ls->h_ligs[p / P->nchain].ntors));
     }
     FeatArgs A;
     A.L = ls->view();
@@ -1072,7 +1099,8 @@ int ud2_features(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, 
     A.sel = sel_d;
     A.n_sel = n_sel;
     A.x = x_d;
-    A.lds_floats_per_wave = lds_prop_floats(apl);
+    A.hdim_ld = odd_ld(dmax);
+    A.lds_floats_per_wave = lds_round4(lds_prop_floats(apl, A.hdim_ld));
     UD2_HIP(launch_features(A, apl, static_cast<hipStream_t>(stream)));
     return UD2_OK;
 }
@@ -1158,15 +1186,16 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
         // While candidates are scored the energy's LDS is idle: the 16 bf16
         // candidate rows (640 floats) take the coordinate / force / BFGS-start
         // area at the slice start, the feature scratch (8 floats per atom) and
-        // the hidden layer (16 x 264 bf16) the Hessian's. Offsets stay even.
-        const int fixed = lds_fixed_floats(apl);
+        // the hidden layer (16 x 264 bf16) the Hessian's. The torsion table
+        // between them stays in use.
+        const int head = lds_head_floats(apl, A.hdim_ld);
         const int a1_floats = kGroup * (kHidden + 8) / 2;
         const int shared = feats ? std::max(dmax * A.hdim_ld, std::max(8 * kWave * apl, a1_floats)) : dmax * A.hdim_ld;
         static_assert(kGroup * kFeat / 2 <= 7 * kWave + kWave * 4 + 8, "candidate rows exceed the idle LDS");
         A.lfeat = 0;
-        A.lscr = fixed;
-        A.la1 = fixed;
-        A.lds_floats_per_wave = (fixed + shared + 1) & ~1;
+        A.lscr = head;
+        A.la1 = head;
+        A.lds_floats_per_wave = lds_round4(head + shared);
         const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
         if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");
         const int64_t items = int64_t(cnt) * P->nchain;

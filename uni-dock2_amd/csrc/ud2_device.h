@@ -263,15 +263,16 @@ struct WPose {
 
 // Per-wave constants of the ligand being processed. The wave's LDS slice
 // (offsets in floats into s_lds) holds: lx float4 coordinates [64*APL], lf
-// pair-force tile [3][64*APL], lo a saved pose, lh the inverse Hessian (row i
-// at lh + i * ld).
+// pair-force tile [3][64*APL], lo a saved pose, lt the ligand's torsion table,
+// lh the inverse Hessian (row i at lh + i * ld).
 template <int APL>
 struct Ctx {
     int natom, ntors, npair, nheavy, dim;
     int atom_off, tors_off, pair_off;
+    int leaf_first;     // the torsion table's order is leaf-first (see apply_update)
     int type[APL];      // XS type; -1 for lanes beyond natom
     int slot[APL];      // grid map of the atom's type
-    int lx, lf, lo, lh, ld;
+    int lx, lf, lo, lt, lh, ld;
     const LigSetDev* L;
     const SceneDev* S;
     int mode;
@@ -285,12 +286,14 @@ __device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const 
     C.natom = d.natom; C.ntors = d.ntors; C.npair = d.npair; C.nheavy = d.nheavy;
     C.dim = 6 + d.ntors;
     C.atom_off = d.atom_off; C.tors_off = d.tors_off; C.pair_off = d.pair_off;
+    C.leaf_first = d.leaf_first;
     C.L = &L; C.S = &S; C.mode = mode;
     C.evals = 0;
     C.lx = base;
     C.lf = base + 4 * kWave * APL;
     C.lo = base + 7 * kWave * APL;
-    C.lh = base + lds_fixed_floats(APL);
+    C.lt = base + lds_fixed_floats(APL);
+    C.lh = base + lds_head_floats(APL, ld);
     C.ld = ld;
 #pragma unroll
     for (int k = 0; k < APL; k++) {
@@ -298,6 +301,52 @@ __device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const 
         C.type[k] = a < d.natom ? __float_as_int(L.atom_x0[d.atom_off + a].w) : -1;
         C.slot[k] = (mode == kGrid && C.type[k] > 0) ? S.slot_tab[C.type[k]] : 0;
     }
+    // torsion table: lane t copies torsion t once, so the kinematics and the gradient read
+    // the torsion tree from LDS instead of global memory on every step
+    if (lane() < d.ntors) {
+        const int tg = d.tors_off + lane();
+        const int2 ax = L.tors_axis[tg];
+        const int2 rr = L.tors_range[tg];
+        const float2 r0 = L.ranges[rr.x];
+        float* e = s_lds + C.lt + lane() * tors_entry_floats(APL);
+        const int word = ax.x | ((ax.y & 0xff) << 8) | ((ax.y >> 16) << 16);  // from | to << 8 | order << 16
+        *reinterpret_cast<float4*>(e) = make_float4(__int_as_float(word), __int_as_float(rr.y), r0.x, r0.y);
+#pragma unroll
+        for (int k = 0; k < APL; k++) {
+            const unsigned long long m = L.tors_mask[(size_t)tg * kMaxApl + k];
+            e[4 + 2 * k] = __uint_as_float(uint32_t(m));
+            e[5 + 2 * k] = __uint_as_float(uint32_t(m >> 32));
+        }
+    }
+    wave_lds_fence();
+}
+
+// Entry t of the torsion table: axis atoms, number of allowed ranges and the first range, and
+// ord, the torsion applied t-th in leaf-first order.
+struct TorsEnt {
+    int from, to, ord, nrange;
+    float lo, hi;
+};
+
+template <int APL>
+__device__ __forceinline__ TorsEnt tors_ent(const Ctx<APL>& C, int t) {
+    const float4 h = *reinterpret_cast<const float4*>(s_lds + C.lt + t * tors_entry_floats(APL));
+    const int ax = __float_as_int(h.x);
+    TorsEnt e;
+    e.from = ax & 0xff;
+    e.to = (ax >> 8) & 0xff;
+    e.ord = ax >> 16;
+    e.nrange = __float_as_int(h.y);
+    e.lo = h.z;
+    e.hi = h.w;
+    return e;
+}
+
+// Rotated-atom mask of torsion t over atom slot k (bit l = atom l + 64 k).
+template <int APL>
+__device__ __forceinline__ unsigned long long tors_mask(const Ctx<APL>& C, int t, int k) {
+    const float2 m = *reinterpret_cast<const float2*>(s_lds + C.lt + t * tors_entry_floats(APL) + 4 + 2 * k);
+    return (unsigned long long)__float_as_uint(m.x) | ((unsigned long long)__float_as_uint(m.y) << 32);
 }
 
 template <int APL>
@@ -389,19 +438,31 @@ __device__ __forceinline__ float clamp_ranges(const LigSetDev& L, int tg, float 
     return best;
 }
 
+// clamp_ranges of torsion t from its table entry: one allowed range inline, more from global memory.
+template <int APL>
+__device__ __forceinline__ float clamp_tors(const Ctx<APL>& C, const TorsEnt& e, int tg, float ang) {
+    if (e.nrange != 1) return clamp_ranges(*C.L, tg, ang);
+    if (ang >= e.lo && ang <= e.hi) return ang;
+    const float dl = fabsf(ang - e.lo), dh = fabsf(ang - e.hi);
+    float best = ang, md = 100.0f;
+    if (dl < md) { md = dl; best = e.lo; }
+    if (dh < md) best = e.hi;
+    return best;
+}
+
 // One torsion step, apply_grad_update_dihe_tile (E/src/cuda/bfgs.cuh:443-527);
 // the centroid is refreshed by the caller after the last torsion (only the
 // final centroid is ever read, bfgs.cuh:531-548, 576-578).
 template <int APL>
 __device__ __forceinline__ void apply_torsion(const Ctx<APL>& C, WPose<APL>& P, int t, float inc_raw) {
     const int tg = C.tors_off + t;
+    const TorsEnt te = tors_ent(C, t);
     const float dt = rdl(P.dihe, t);
     float inc = normalize_angle(inc_raw);
-    const float nw = uni(clamp_ranges(*C.L, tg, normalize_angle(dt + inc)));
+    const float nw = uni(clamp_tors(C, te, tg, normalize_angle(dt + inc)));
     inc = nw - dt;
     if (lane() == t) P.dihe = nw;
-    const int2 ax = C.L->tors_axis[tg];
-    const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
+    const float3 xt = atom_pos(P, te.to), xf = atom_pos(P, te.from);
     float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
     const float l = fsqrt(ux * ux + uy * uy + uz * uz);
     if (l <= kEps) { ux = 1.0f; uy = 0.0f; uz = 0.0f; }
@@ -410,10 +471,9 @@ __device__ __forceinline__ void apply_torsion(const Ctx<APL>& C, WPose<APL>& P, 
     __sincosf(inc, &s, &c);
     const float rn = frsq(c * c + s * s);  // unit (c, s): no growth of the rotated fragment per step
     ux = uni(ux); uy = uni(uy); uz = uni(uz); s = uni(s * rn); c = uni(c * rn);
-    const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
 #pragma unroll
     for (int k = 0; k < APL; k++) {
-        const unsigned long long mk = mask[k];
+        const unsigned long long mk = tors_mask(C, t, k);
         if ((mk >> lane()) & 1ull) {
             const float vx = P.x[k] - xt.x, vy = P.y[k] - xt.y, vz = P.z[k] - xt.z;
             const float m = (1.0f - c) * (ux * vx + uy * vy + uz * vz);
@@ -442,13 +502,89 @@ __device__ __forceinline__ void rigid_move(const Ctx<APL>& C, WPose<APL>& P, flo
     }
 }
 
-// apply_grad_update_pose_tile (bfgs.cuh:560-618): torsions in index order, then
-// centre += alpha * dir[0..2] and a rotation by q(alpha * dir[3..5]) about the
-// old centre; rot_vec := alpha * dir[3..5]. dir is lane-distributed.
+// Position of atom a (lane a % 64, slot a / 64) for every lane, each with its own a.
+template <int APL>
+__device__ __forceinline__ float3 lane_gather(const WPose<APL>& P, int a) {
+    float3 r = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < APL; k++) {
+        const float x = __shfl(P.x[k], a & 63), y = __shfl(P.y[k], a & 63), z = __shfl(P.z[k], a & 63);
+        if ((a >> 6) == k) r = make_float3(x, y, z);
+    }
+    return r;
+}
+
+// All torsion steps of one pose update, leaf first. A torsion's pivot and axis atoms are
+// moved only by its ancestors, which come later, so lane t first computes torsion t's step
+// from the coordinates before the update: the angle normalised and clamped into the allowed
+// ranges (as apply_torsion), the pivot, the unit axis and (cos, sin). The rotations then run
+// with register operands only: no step waits for the one before it.
+template <int APL>
+__device__ __forceinline__ void apply_torsions_leaf_first(const Ctx<APL>& C, WPose<APL>& P, float dir, float alpha) {
+    const int ln = lane();
+    const float inc_raw = __shfl(dir, (ln + 6) & 63) * alpha;  // torsion components sit on lanes 6 + t
+    // every lane reads an entry (lanes past ntors read entry 0) and gathers its axis atoms
+    const TorsEnt te = tors_ent(C, ln < C.ntors ? ln : 0);
+    const float3 xt = lane_gather(P, te.to), xf = lane_gather(P, te.from);
+    float px = 0.0f, py = 0.0f, pz = 0.0f, ux = 1.0f, uy = 0.0f, uz = 0.0f, c = 1.0f, s = 0.0f;
+    int ord = 0;
+    uint32_t mlo[APL], mhi[APL];
+#pragma unroll
+    for (int k = 0; k < APL; k++) mlo[k] = mhi[k] = 0u;
+    if (ln < C.ntors) {
+        ord = te.ord;
+        float inc = normalize_angle(inc_raw);
+        const float nw = clamp_tors(C, te, C.tors_off + ln, normalize_angle(P.dihe + inc));
+        inc = nw - P.dihe;
+        P.dihe = nw;
+        px = xt.x; py = xt.y; pz = xt.z;
+        const float ax = xt.x - xf.x, ay = xt.y - xf.y, az = xt.z - xf.z;
+        const float l = fsqrt(ax * ax + ay * ay + az * az);
+        if (l > kEps) { const float il = frcp(l); ux = ax * il; uy = ay * il; uz = az * il; }
+        __sincosf(inc, &s, &c);
+        const float rn = frsq(c * c + s * s);
+        s *= rn;
+        c *= rn;
+#pragma unroll
+        for (int k = 0; k < APL; k++) {
+            const unsigned long long m = tors_mask(C, ln, k);
+            mlo[k] = uint32_t(m);
+            mhi[k] = uint32_t(m >> 32);
+        }
+    }
+#pragma unroll 1
+    for (int q = 0; q < C.ntors; q++) {
+        const int t = rdl_i(ord, q);
+        const float tx = rdl(px, t), ty = rdl(py, t), tz = rdl(pz, t);
+        const float vx0 = rdl(ux, t), vy0 = rdl(uy, t), vz0 = rdl(uz, t), ct = rdl(c, t), st = rdl(s, t);
+#pragma unroll
+        for (int k = 0; k < APL; k++) {
+            const uint32_t wlo = uint32_t(rdl_i(int(mlo[k]), t)), whi = uint32_t(rdl_i(int(mhi[k]), t));
+            const uint32_t w = ln < 32 ? wlo : whi;
+            if ((w >> (ln & 31)) & 1u) {
+                const float vx = P.x[k] - tx, vy = P.y[k] - ty, vz = P.z[k] - tz;
+                const float m = (1.0f - ct) * (vx0 * vx + vy0 * vy + vz0 * vz);
+                const float cxv = vy0 * vz - vz0 * vy, cyv = vz0 * vx - vx0 * vz, czv = vx0 * vy - vy0 * vx;
+                P.x[k] = ct * vx + m * vx0 + st * cxv + tx;
+                P.y[k] = ct * vy + m * vy0 + st * cyv + ty;
+                P.z[k] = ct * vz + m * vz0 + st * czv + tz;
+            }
+        }
+    }
+}
+
+// apply_grad_update_pose_tile (bfgs.cuh:560-618): torsions (leaf first when the
+// ligand's torsion tree allows it, else in index order), then centre += alpha *
+// dir[0..2] and a rotation by q(alpha * dir[3..5]) about the old centre;
+// rot_vec := alpha * dir[3..5]. dir is lane-distributed.
 template <int APL>
 __device__ __forceinline__ void apply_update(const Ctx<APL>& C, WPose<APL>& P, float dir, float alpha) {
+    if (C.leaf_first) {
+        if (C.ntors > 0) apply_torsions_leaf_first(C, P, dir, alpha);
+    } else {
 #pragma unroll 1
-    for (int t = 0; t < C.ntors; t++) apply_torsion(C, P, t, rdl(dir, 6 + t) * alpha);
+        for (int t = 0; t < C.ntors; t++) apply_torsion(C, P, t, rdl(dir, 6 + t) * alpha);
+    }
     if (C.ntors > 0) centroid(C, P);
     if (C.L->constraint) return;  // constraint docking: no rigid step (bfgs.cuh:571)
     const float o0 = P.c0, o1 = P.c1, o2 = P.c2;
@@ -580,9 +716,12 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
     // intra-molecular pairs with the per-pair curl; each pair once
     const uint2* pr = C.L->pairs + C.pair_off;
     const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
+    // the next pair record is loaded while the current one is scored
+    uint2 rec_next = ln < C.npair ? pr[ln] : make_uint2(0u, 0u);
 #pragma unroll 1
     for (int p = ln; p < C.npair; p += kWave) {
-        const uint2 rec = pr[p];
+        const uint2 rec = rec_next;
+        if (p + kWave < C.npair) rec_next = pr[p + kWave];
         const int i = rec.x & 255, j = (rec.x >> 8) & 255, fl = (rec.x >> 16) & 3;
         const float rs = __uint_as_float(rec.y);
         const float4 xi = lx4[i], xj = lx4[j];
@@ -653,17 +792,15 @@ __device__ __forceinline__ float pose_grad(const Ctx<APL>& C, const WPose<APL>& 
     }
 #pragma unroll 1
     for (int t = 0; t < C.ntors; t++) {
-        const int tg = C.tors_off + t;
-        const int2 ax = C.L->tors_axis[tg];
-        const float3 xt = atom_pos(P, ax.y), xf = atom_pos(P, ax.x);
+        const TorsEnt te = tors_ent(C, t);
+        const float3 xt = atom_pos(P, te.to), xf = atom_pos(P, te.from);
         float ux = xt.x - xf.x, uy = xt.y - xf.y, uz = xt.z - xf.z;
         const float inv = frsq(ux * ux + uy * uy + uz * uz);
         ux = uni(ux * inv); uy = uni(uy * inv); uz = uni(uz * inv);
-        const unsigned long long* mask = C.L->tors_mask + (size_t)tg * kMaxApl;
         float st = 0.0f;
 #pragma unroll
         for (int k = 0; k < APL; k++) {
-            if (((mask[k] >> ln) & 1ull) && heavy(C, k)) {
+            if (((tors_mask(C, t, k) >> ln) & 1ull) && heavy(C, k)) {
                 const float lx = P.x[k] - xt.x, ly = P.y[k] - xt.y, lz = P.z[k] - xt.z;
                 const float cx = ly * fz[k] - lz * fy[k], cy = lz * fx[k] - lx * fz[k], cz = lx * fy[k] - ly * fx[k];
                 st += cx * ux + cy * uy + cz * uz;
@@ -868,18 +1005,18 @@ __device__ __forceinline__ Move mutate(const Ctx<APL>& C, WPose<APL>& P, Rng& rn
     } else {
         const int t = which - 2;
         const uint4 bb = rng.next();
-        const int2 rr = C.L->tors_range[C.tors_off + t];
-        int i = int(ceilf(__fsub_rn(__fmul_rn(u01(bb.x), float(rr.y)), 1.0f)));
-        i = i < 0 ? 0 : (i > rr.y - 1 ? rr.y - 1 : i);
-        const float2 r = C.L->ranges[rr.x + i];
+        const TorsEnt te = tors_ent(C, t);
+        int i = int(ceilf(__fsub_rn(__fmul_rn(u01(bb.x), float(te.nrange)), 1.0f)));
+        i = i < 0 ? 0 : (i > te.nrange - 1 ? te.nrange - 1 : i);
+        float2 r = make_float2(te.lo, te.hi);
+        if (i > 0) r = C.L->ranges[C.L->tors_range[C.tors_off + t].x + i];
         const float target = r.x + (r.y - r.x) * u01(bb.y);
         const float inc = target - rdl(P.dihe, t);
         apply_torsion(C, P, t, inc);
         centroid(C, P);
-        const unsigned long long* mask = C.L->tors_mask + (size_t)(C.tors_off + t) * kMaxApl;
         int nmoved = 0;
 #pragma unroll
-        for (int k = 0; k < APL; k++) nmoved += __popcll(mask[k] & __ballot(heavy(C, k)));
+        for (int k = 0; k < APL; k++) nmoved += __popcll(tors_mask(C, t, k) & __ballot(heavy(C, k)));
         mv.kind = 2;
         mv.mag = fabsf(normalize_angle(inc)) * (1.0f / kPi);
         mv.moved = C.nheavy > 0 ? float(nmoved) / float(C.nheavy) : 0.0f;

@@ -21,13 +21,14 @@ struct DevLig {
     int32_t tors_off;   // into tors_* (torsions)
     int32_t pair_off;   // into pairs
     int32_t apl;        // ceil(natom / 64)
+    int32_t leaf_first; // 1: the torsions have a leaf-first application order (tors_axis .y >> 16)
 };
 
 // Device view of a ligand set.
 struct LigSetDev {
     const DevLig* ligs;
     const float4* atom_x0;                 // x, y, z, type (int bits)
-    const int2* tors_axis;                 // (from, to)
+    const int2* tors_axis;                 // (from, to | order << 16); entry t's order = the torsion applied t-th, leaf first
     const float* tors_dihe0;               // input dihedral (radians)
     const int2* tors_range;                // (global offset into ranges, count)
     const float2* ranges;                  // (lo, hi) radians
@@ -116,6 +117,7 @@ struct PoseVecArgs {
     float v_cap, slope;
     int32_t mode;
     int32_t lds_floats_per_wave;
+    int32_t hdim_ld;   // sizes the torsion table (>= 6 + ntors)
 };
 
 struct ClusterArgs {
@@ -223,6 +225,7 @@ struct FeatArgs {
     int32_t n_sel;
     uint16_t* x;             // [n_sel][80]
     int32_t lds_floats_per_wave;
+    int32_t hdim_ld;         // sizes the torsion table (>= 6 + ntors of every selected ligand)
 };
 
 // Launchers (ud2_kernels.hip); each returns a hipError_t.
@@ -234,12 +237,22 @@ hipError_t launch_cluster(const ClusterArgs& a, hipStream_t s);
 hipError_t launch_final_score(const ScoreArgs& a, int apl, bool decomp, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 int mc_blocks_per_cu(int apl, int lds_bytes_per_block);
-// LDS floats of a wave slice before its Hessian tile.
-// coordinates + pair forces (7 * 64 * apl) and two parked poses (64 * (3 * apl + 1) + 8 each)
+// LDS floats of a wave slice before its torsion table.
+// coordinates + pair forces (7 * 64 * apl) and two parked poses (64 * (3 * apl + 1) + 8 each);
+// a multiple of 4, so the table after it stays 16-byte aligned
 __host__ __device__ constexpr int lds_fixed_floats(int apl) { return 7 * kWave * apl + 2 * (kWave * (3 * apl + 1) + 8); }
+// Torsion table of the wave's ligand, filled once per work item: per torsion {from | to << 16,
+// number of allowed ranges, first range lo, hi} and the rotated-atom mask word pairs of each
+// atom slot, padded to 16 bytes. ld (the Hessian's leading dimension) >= 6 + ntors sizes it.
+__host__ __device__ constexpr int tors_entry_floats(int apl) { return apl <= 2 ? 8 : 12; }
+__host__ __device__ constexpr int lds_tors_floats(int apl, int ld) { return (ld > 6 ? ld - 6 : 0) * tors_entry_floats(apl); }
+// LDS floats of a wave slice before its Hessian tile (or feature scratch).
+__host__ __device__ constexpr int lds_head_floats(int apl, int ld) { return lds_fixed_floats(apl) + lds_tors_floats(apl, ld); }
+// Per-wave slices are rounded to 4 floats so every slice starts 16-byte aligned.
+__host__ __device__ constexpr int lds_round4(int n) { return (n + 3) & ~3; }
 int posevec_blocks_per_cu(int apl, int lds_bytes_per_block);
-// LDS floats per wave of the proposal kernel: the fixed slice and 8 floats per atom of feature scratch.
-__host__ __device__ constexpr int lds_prop_floats(int apl) { return lds_fixed_floats(apl) + 8 * kWave * apl; }
+// LDS floats per wave of the features kernel: the head and 8 floats per atom of feature scratch.
+__host__ __device__ constexpr int lds_prop_floats(int apl, int ld) { return lds_head_floats(apl, ld) + 8 * kWave * apl; }
 
 // Learned proposal (ud2_learned.hip) and network (ud2_mlp.hip) launchers.
 hipError_t launch_lmc(LmcArgs a, int apl, int n_blocks, hipStream_t s);

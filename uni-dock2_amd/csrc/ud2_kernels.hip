@@ -11,8 +11,11 @@ namespace ud2 {
 // and runs one whole chain of mc_kernel (mc.cu:275-398), randomisation first.
 // The best pose goes to the pose buffers whenever it improves (the reference
 // copies it to out_pose at the same points, mc.cu:389-392).
+// One-atom-per-lane ligands fit four waves per SIMD in LDS; the occupancy hint
+// holds the register allocation to the 128 VGPRs that allows.
 template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) void mc_kernel(McArgs A) {
+__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(APL == 1 ? 4 : 1)))
+void mc_kernel(McArgs A) {
     const int w = threadIdx.x / kWave;
     const int base = w * A.lds_floats_per_wave;
     const int nchain = A.P.nchain;
@@ -131,7 +134,7 @@ template <int APL>
 __global__ __launch_bounds__(kWave* kWavesPerBlock) void posevec_kernel(PoseVecArgs A) {
     const int w = threadIdx.x / kWave;
     Ctx<APL> C;
-    load_ctx(C, A.L, A.S, A.lig, A.mode, w * A.lds_floats_per_wave, 1);
+    load_ctx(C, A.L, A.S, A.lig, A.mode, w * A.lds_floats_per_wave, A.hdim_ld);
     WPose<APL> P0;
     initial_pose(C, A.L, A.lig, P0);
     const int ln = lane();

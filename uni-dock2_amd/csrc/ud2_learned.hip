@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kWave* kWavesPerBlock) void features_kernel(FeatArg
     const int pid = A.sel[s];
     const int lig = pid / A.P.nchain, chain = pid % A.P.nchain;
     Ctx<APL> C;
-    load_ctx(C, A.L, A.S, lig, kGrid, w * A.lds_floats_per_wave, 1);
+    load_ctx(C, A.L, A.S, lig, kGrid, w * A.lds_floats_per_wave, A.hdim_ld);
     WPose<APL> p;
     load_pose<APL>(A.L, A.P, lig, chain, p);
     Move mv;
