@@ -845,6 +845,27 @@ __device__ __forceinline__ void line_search(const Ctx<APL>& C, const WPose<APL>&
     alpha_out = alpha;
 }
 
+__device__ __forceinline__ int row_lane_clamped(int ln, int dim) { return ln < dim ? ln : dim - 1; }
+
+// Row of the inverse Hessian (at LDS offset hrow) times the lane-distributed vector v over
+// dim components: four columns per pass with independent partial sums, so their LDS reads
+// are in flight together. Every lane reads a real row (callers mask lanes past dim).
+template <int APL>
+__device__ __forceinline__ float hess_row_dot(const Ctx<APL>& C, int hrow, int dim, float v) {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    int j = 0;
+    for (; j + 4 <= dim; j += 4) {
+        const float v0 = rdl(v, j), v1 = rdl(v, j + 1), v2 = rdl(v, j + 2), v3 = rdl(v, j + 3);
+        const float* h = s_lds + hrow + j;
+        s0 += h[0] * v0;
+        s1 += h[1] * v1;
+        s2 += h[2] * v2;
+        s3 += h[3] * v3;
+    }
+    for (; j < dim; j++) s0 += s_lds[hrow + j] * rdl(v, j);
+    return (s0 + s1) + (s2 + s3);
+}
+
 // bfgs_tile (bfgs.cuh:729-829): the inverse Hessian lives in the wave's LDS
 // tile (lane i owns row i) and the start pose is parked in LDS for the
 // restore-on-failure at the end.
@@ -852,6 +873,7 @@ template <int APL>
 __device__ __forceinline__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_steps, float v_cap, float slope) {
     const int ln = lane(), dim = C.dim;
     const int hrow = C.lh + ln * C.ld;
+    const int hrow_rd = C.lh + (row_lane_clamped(ln, dim)) * C.ld;  // lanes past dim read the last row
     const bool row = ln < dim;
     float g;
     const float E_ori = energy_grad(C, x, v_cap, slope, g);
@@ -864,11 +886,7 @@ __device__ __forceinline__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_s
     float p = g;
     for (int step = 0; step < max_steps; step++) {
         // readlane stays in wave-uniform control flow; rows beyond dim are masked
-        float s = 0.0f;
-        for (int j = 0; j < dim; j++) {
-            const float gj = rdl(g, j);
-            if (row) s += s_lds[hrow + j] * gj;
-        }
+        const float s = hess_row_dot(C, hrow_rd, dim, g);
         p = row ? -s : 0.0f;
         line_search(C, x, g, p, E, v_cap, slope, x_new, g_new, E1, alpha);
         const float y = g_new - g;
@@ -886,16 +904,26 @@ __device__ __forceinline__ void bfgs(const Ctx<APL>& C, WPose<APL>& x, int max_s
         }
         const float yp = uni(wave_sum(y * p));
         if (alpha * yp < kEps) continue;
-        float hy = 0.0f;
-        for (int j = 0; j < dim; j++) {
-            const float yj = rdl(y, j);
-            if (row) hy += s_lds[hrow + j] * yj;
-        }
+        const float hy = hess_row_dot(C, hrow_rd, dim, y);
         const float mhy = row ? -hy : 0.0f;
         const float yhy = -wave_sum(y * mhy);
         const float r = 1.0f / (alpha * yp);
         const float a1 = uni(alpha * r), a2 = uni(alpha * alpha * (r * r * yhy + r));
-        for (int j = 0; j < dim; j++) {
+        // four columns per pass: their reads are in flight together
+        int j = 0;
+        for (; j + 4 <= dim; j += 4) {
+            const float p0 = rdl(p, j), p1 = rdl(p, j + 1), p2 = rdl(p, j + 2), p3 = rdl(p, j + 3);
+            const float m0 = rdl(mhy, j), m1 = rdl(mhy, j + 1), m2 = rdl(mhy, j + 2), m3 = rdl(mhy, j + 3);
+            if (row) {
+                float* h = s_lds + hrow + j;
+                const float h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+                h[0] = h0 + (a1 * (mhy * p0 + m0 * p) + a2 * p * p0);
+                h[1] = h1 + (a1 * (mhy * p1 + m1 * p) + a2 * p * p1);
+                h[2] = h2 + (a1 * (mhy * p2 + m2 * p) + a2 * p * p2);
+                h[3] = h3 + (a1 * (mhy * p3 + m3 * p) + a2 * p * p3);
+            }
+        }
+        for (; j < dim; j++) {
             const float pj = rdl(p, j), mj = rdl(mhy, j);
             if (row) s_lds[hrow + j] += a1 * (mhy * pj + mj * p) + a2 * p * pj;
         }
