@@ -294,3 +294,46 @@ def test_energy_decomp_through_the_pipeline(gpu, tmp_path):
         # the JSON keeps 3 decimals of every coordinate and term (truncated): 1e-2 absolute
         assert np.all(np.abs(d - dref) <= 1e-2 + 1e-3 * np.abs(dref))
         assert abs(d.sum() - p["energy"][3]) < 0.25
+
+
+@pytest.mark.parametrize("option", ["constraint", "pos"])
+def test_learned_search_with_option(gpu, scene, receptor, box, lig_b, option):
+    """The learned kernel (packaged network, 4 randomised starts pre-screened by the network, 16
+    candidates sampled at T = 1) under constraint docking or with bias sites: constraint keeps the
+    root fragment bit-exact through the start pre-screen, mutations and BFGS; with bias, every stored
+    energy is the exact biased search energy of its pose (V_CAP 1000), re-evaluated separately."""
+    import torch
+    from unidock_engine import proposal as LP
+    print(f"learned search, {option}", flush=True)
+    kw = dict(constraint=True) if option == "constraint" else dict(bias="pos", bias_k=1.0)
+    ls = gpu.LigSet([lig_b], **kw)
+    nchain = 64
+    pb = gpu.PoseBuffers(ls, nchain)
+    pb.init()
+    mlp = gpu.Mlp(LP.load())
+    lp = gpu.LmcParams()
+    lp.mc = gpu.McParams(mc_steps=4, opt_steps=8, randomize=1, seed=51, mode=gpu.MODE_GRID, lig_index_base=0,
+                         stats_d=None)
+    lp.policy, lp.n_cand, lp.top_k, lp.sel_temp, lp.n_start = gpu.POLICY_LEARNED, 16, 1, 1.0, 4
+    gpu.check(gpu.lib().ud2_lmc(scene.h, ls.h, pb.ref, C.byref(lp), mlp.h, gpu.stream_ptr()), "ud2_lmc")
+    torch.cuda.synchronize()
+    e = pb.e[:nchain].cpu().numpy()
+    assert np.isfinite(e).all()
+    if option == "constraint":
+        root = root_atoms(lig_b)
+        for c in range(nchain):
+            x = pb.pose(0, c)["x"].reshape(-1, 3).astype(np.float32)
+            assert np.array_equal(x[root], lig_b.coords[root]), c
+    else:
+        sel = torch.arange(nchain, dtype=torch.int32, device="cuda")
+        e2 = torch.zeros(nchain, device="cuda")
+        g = torch.zeros((nchain, 64), device="cuda")
+        gpu.check(gpu.lib().ud2_energy_grad(scene.h, ls.h, pb.ref, dptr(sel), nchain, gpu.MODE_GRID, 1000.0, 1e6,
+                                            dptr(e2), dptr(g), gpu.stream_ptr()))
+        torch.cuda.synchronize()
+        e2 = e2.cpu().numpy()
+        assert np.all(np.abs(e2 - e) <= 1e-4 * np.maximum(1.0, np.abs(e))), np.abs(e2 - e).max()
+        topo = O.Topo(lig_b, bias="pos", bias_k=1.0)
+        osc = O.Scene(receptor, box, "direct")
+        fs = [O.final_score(topo, pb.pose(0, c)["x"], osc)[3] for c in range(nchain)]
+        assert min(fs) < -0.1  # the wells are felt by the kept poses
