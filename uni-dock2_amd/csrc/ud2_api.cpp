@@ -69,7 +69,8 @@ struct ud2_scene {
     float4* d_rec = nullptr;
     int32_t nrec = 0;
     float box_lo[3], box_hi[3];
-    float* d_maps = nullptr;
+    float* d_maps = nullptr;     // [types][nz][ny][nx]
+    float4* d_cells = nullptr;   // cell-major copy the kernels gather from
     int32_t gn[3] = {0, 0, 0};
     float gorigin[3] = {0, 0, 0};
     float gspacing = 0;
@@ -88,7 +89,7 @@ struct ud2_scene {
             s.gn[d] = gn[d];
             s.gorigin[d] = gorigin[d];
         }
-        s.maps = d_maps;
+        s.cells = d_cells;
         s.gspacing = gspacing;
         s.ginv = gspacing > 0 ? 1.0f / gspacing : 0.0f;
         s.slot_tab = d_slot;
@@ -340,6 +341,25 @@ static int set_probe_types(ud2_scene* sc, const int32_t* probe_types_h, int32_t 
     return upload(&sc->d_slot, std::vector<int32_t>(sc->slot_of_type, sc->slot_of_type + kNumTypes));
 }
 
+namespace {
+
+// (Re)builds the cell-major copy of the scene's maps.
+int make_cells(ud2_scene* sc) {
+    if (sc->d_cells) UD2_HIP(hipFree(sc->d_cells));
+    sc->d_cells = nullptr;
+    const size_t ncell = size_t(sc->gn[0] - 1) * (sc->gn[1] - 1) * (sc->gn[2] - 1) * sc->probe_types.size();
+    UD2_HIP(hipMalloc(reinterpret_cast<void**>(&sc->d_cells), std::max<size_t>(ncell, 1) * 2 * sizeof(float4)));
+    BrickArgs B;
+    B.maps = sc->d_maps;
+    B.cells = sc->d_cells;
+    for (int d = 0; d < 3; d++) B.gn[d] = sc->gn[d];
+    B.nslot = int32_t(sc->probe_types.size());
+    UD2_HIP(launch_brick(B, nullptr));
+    return UD2_OK;
+}
+
+}  // namespace
+
 int ud2_scene_build_grids(ud2_scene* sc, float spacing, const int32_t* probe_types_h, int32_t n_types) {
     if (!sc) return fail(UD2_ERR_ARG, "null scene");
     if (!(spacing > 0.0f)) return fail(UD2_ERR_ARG, "spacing must be positive");
@@ -363,6 +383,8 @@ int ud2_scene_build_grids(ud2_scene* sc, float spacing, const int32_t* probe_typ
     A.nslot = n_types;
     A.maps = sc->d_maps;
     UD2_HIP(launch_grid_build(A, nullptr));
+    rc = make_cells(sc);
+    if (rc) return rc;
     UD2_HIP(hipDeviceSynchronize());
     return UD2_OK;
 }
@@ -386,6 +408,9 @@ int ud2_scene_set_grids(ud2_scene* sc, const int32_t n_h[3], const float origin_
     size_t bytes = size_t(npt) * n_types * sizeof(float);
     UD2_HIP(hipMalloc(reinterpret_cast<void**>(&sc->d_maps), bytes));
     UD2_HIP(hipMemcpy(sc->d_maps, maps_h, bytes, hipMemcpyHostToDevice));
+    rc = make_cells(sc);
+    if (rc) return rc;
+    UD2_HIP(hipDeviceSynchronize());
     return UD2_OK;
 }
 
@@ -413,6 +438,7 @@ int ud2_scene_destroy(ud2_scene* sc) {
     if (!sc) return UD2_OK;
     if (sc->d_rec) hipFree(sc->d_rec);
     if (sc->d_maps) hipFree(sc->d_maps);
+    if (sc->d_cells) (void)hipFree(sc->d_cells);
     if (sc->d_probe) (void)hipFree(sc->d_probe);
     if (sc->d_slot) (void)hipFree(sc->d_slot);
     delete sc;

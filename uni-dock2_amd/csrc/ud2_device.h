@@ -234,10 +234,12 @@ __device__ __forceinline__ float grid_eval(const SceneDev& S, int slot, float px
         i0[d] = i;
         fr[d] = v - float(i);
     }
-    const int nx = S.gn[0], nxy = S.gn[0] * S.gn[1];
-    const float* m = S.maps + (size_t)slot * (size_t)nxy * S.gn[2] + (size_t)i0[2] * nxy + i0[1] * nx + i0[0];
-    float c000 = m[0], c100 = m[1], c010 = m[nx], c110 = m[nx + 1];
-    float c001 = m[nxy], c101 = m[nxy + 1], c011 = m[nxy + nx], c111 = m[nxy + nx + 1];
+    // the cell's 8 corners: one 32-byte sector of the cell-major copy
+    const int cx = S.gn[0] - 1, cxy = cx * (S.gn[1] - 1);
+    const float4* m = S.cells + 2 * ((size_t)slot * (size_t)cxy * (S.gn[2] - 1) + (size_t)i0[2] * cxy + i0[1] * cx + i0[0]);
+    const float4 lo = m[0], hi = m[1];
+    const float c000 = lo.x, c100 = lo.y, c010 = lo.z, c110 = lo.w;
+    const float c001 = hi.x, c101 = hi.y, c011 = hi.z, c111 = hi.w;
     float fx = fr[0], fy = fr[1], fz = fr[2], hx = 1.0f - fx, hy = 1.0f - fy, hz = 1.0f - fz;
     float v00 = c000 * hx + c100 * fx, v10 = c010 * hx + c110 * fx;
     float v01 = c001 * hx + c101 * fx, v11 = c011 * hx + c111 * fx;

@@ -49,7 +49,7 @@ struct SceneDev {
     const float4* rec;  // heavy receptor atoms: x, y, z, type (int bits)
     int32_t nrec;
     float box_lo[3], box_hi[3];
-    const float* maps;  // [slot][nz][ny][nx]
+    const float4* cells;  // maps cell-major: [slot][cz][cy][cx] x {c000 c100 c010 c110}, {c001 c101 c011 c111}
     int32_t gn[3];
     float gorigin[3];
     float gspacing, ginv;
@@ -150,6 +150,15 @@ struct GridArgs {
     float* maps;
 };
 
+// Cell-major copy of the maps (SceneDev::cells): the 8 corners of a grid cell are one
+// aligned 32-byte sector, so a trilinear gather reads two float4 instead of 8 scattered floats.
+struct BrickArgs {
+    const float* maps;  // [nslot][nz][ny][nx]
+    float4* cells;      // [nslot][nz-1][ny-1][nx-1][2]
+    int32_t gn[3];
+    int32_t nslot;
+};
+
 // ---------------------------------------------------------- learned proposal
 constexpr int kFeat = 80;     // network inputs of one candidate
 constexpr int kPooled = 64;   // 8 feature classes x 8 statistics
@@ -230,6 +239,7 @@ struct FeatArgs {
 
 // Launchers (ud2_kernels.hip); each returns a hipError_t.
 hipError_t launch_grid_build(const GridArgs& a, hipStream_t s);
+hipError_t launch_brick(const BrickArgs& a, hipStream_t s);
 hipError_t launch_mc(McArgs a, int apl, int n_blocks, hipStream_t s);
 hipError_t launch_opt(OptArgs a, int apl, hipStream_t s);
 hipError_t launch_posevec(PoseVecArgs a, int apl, int n_blocks, hipStream_t s);

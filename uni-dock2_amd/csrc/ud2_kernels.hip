@@ -383,6 +383,28 @@ hipError_t launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Cell-major copy of the maps: one thread per cell and map.
+__global__ __launch_bounds__(256) void brick_kernel(BrickArgs A) {
+    const int cx = A.gn[0] - 1, cy = A.gn[1] - 1, cz = A.gn[2] - 1;
+    const int64_t ncell = (int64_t)cx * cy * cz;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell * A.nslot) return;
+    const int slot = int(id / ncell);
+    const int64_t c = id % ncell;
+    const int ix = int(c % cx), iy = int((c / cx) % cy), iz = int(c / ((int64_t)cx * cy));
+    const int nx = A.gn[0], nxy = A.gn[0] * A.gn[1];
+    const float* m = A.maps + (size_t)slot * (size_t)nxy * A.gn[2] + (size_t)iz * nxy + (size_t)iy * nx + ix;
+    A.cells[2 * id] = make_float4(m[0], m[1], m[nx], m[nx + 1]);
+    A.cells[2 * id + 1] = make_float4(m[nxy], m[nxy + 1], m[nxy + nx], m[nxy + nx + 1]);
+}
+
+hipError_t launch_brick(const BrickArgs& a, hipStream_t s) {
+    const int64_t n = (int64_t)(a.gn[0] - 1) * (a.gn[1] - 1) * (a.gn[2] - 1) * a.nslot;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(brick_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_grid_build(const GridArgs& a, hipStream_t s) {
     int64_t npt = (int64_t)a.S.gn[0] * a.S.gn[1] * a.S.gn[2];
     hipLaunchKernelGGL(grid_kernel, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, s, a);
