@@ -185,45 +185,59 @@ class DockingPipeline:
 
     # ------------------------------------------------------------------
     def _run(self) -> None:
-        p = dict(self.params)
         # the reference serialises the cached objects to JSON text and parses them back
         doc = json.loads(json.dumps(self.json_data))
-        if p["bias"] not in ("no", "pos", "align"):  # apply_bias, core.cpp:86-97
-            raise ValueError(f"Not supported bias: {p['bias']} doesn't belong to (no, pos, align)")
-        apply_search_mode(p)
-        if p["opt_steps"] < 0:
-            p["opt_steps"] = -1
-        if p["constraint_docking"]:  # prepare_context_by_input, core.cpp:142-146
-            p["randomize"] = False
-        apply_task(p)
+        self.output_files.extend(core_pipeline(dict(self.params), self.box, doc, self.inter_mode))
+
+
+def core_pipeline(p: dict, box: Box, doc: dict, inter_mode: int = N.MODE_GRID) -> list:
+    """core_pipeline, src/screening/core.cpp:100-223, over an engine-JSON document (receptor under
+    "receptor", one member per ligand, optional "score"): bias check, search-mode and task presets,
+    size groups and batches of at most 1,000 ligands, one docking run and one output file per batch.
+    ``p`` holds DockingPipeline's keyword arguments. Returns the files written; raises on failure
+    (both callers turn that into the reference's failure convention)."""
+    p = dict(p)
+    if p["bias"] not in ("no", "pos", "align"):  # apply_bias, core.cpp:86-97
+        raise ValueError(f"Not supported bias: {p['bias']} doesn't belong to (no, pos, align)")
+    apply_search_mode(p)
+    if p["opt_steps"] < 0:
+        p["opt_steps"] = -1
+    if p["constraint_docking"]:  # prepare_context_by_input, core.cpp:142-146
+        p["randomize"] = False
+    apply_task(p)
+    if p["output_dir"]:
         os.makedirs(p["output_dir"], exist_ok=True)
-        N.check(N.lib().ud2_set_device(p["gpu_device_id"]), "ud2_set_device")
-        receptor = receptor_from_json(doc.get("receptor", []), self.box)
-        ligands = [ligand_from_json(k, v, p["use_tor_lib"]) for k, v in doc.items()
-                   if k not in ("score", "receptor")]
-        if not ligands:
-            log.error("No flexible molecules are found")
-            return
-        eng = DockEngine(receptor, self.box, mode=self.inter_mode)
-        eng.ensure_types(ligands)
-        batch_id = 0
-        opt_steps = p["opt_steps"]
-        for group in group_ligands(ligands):
-            for start in range(0, len(group), MAX_BATCH):
-                idx = group[start:start + MAX_BATCH]
-                batch = [ligands[i] for i in idx]
-                batch_id += 1
-                if opt_steps == -1:  # heuristic of the first batch persists (screening.cpp:206-208)
-                    opt_steps = heuristic_opt_steps(max(l.natom for l in batch))
-                prm = DockParams(nchain=p["exhaustiveness"], mc_steps=p["mc_steps"], opt_steps=opt_steps,
-                                 refine_steps=p["refine_steps"], num_pose=p["num_pose"],
-                                 rmsd_limit=p["rmsd_limit"], energy_range=p["energy_range"], seed=p["seed"],
-                                 randomize=p["randomize"], mode=self.inter_mode, energy_decomp=p["energy_decomp"])
-                t0 = time.time()
-                ls = N.LigSet(batch, bias=p["bias"], bias_k=p["bias_k"], constraint=p["constraint_docking"])
-                results, _ = eng.dock(ls, prm)
-                path = os.path.join(p["output_dir"], f"{p['name_json']}_{batch_id}.json")
-                write_json_with_retry(path, dumps_poses(results))
-                self.output_files.append(path)
-                log.info("Batch %d (%d ligands) running time: %.1f ms", batch_id, len(batch),
-                         (time.time() - t0) * 1e3)
+    N.check(N.lib().ud2_set_device(p["gpu_device_id"]), "ud2_set_device")
+    receptor = receptor_from_json(doc.get("receptor", []), box)
+    ligands = [ligand_from_json(k, v, p["use_tor_lib"]) for k, v in doc.items()
+               if k not in ("score", "receptor")]
+    if not ligands:
+        log.error("No flexible molecules are found")
+        return []
+    eng = DockEngine(receptor, box, mode=inter_mode)
+    eng.ensure_types(ligands)
+    files = []
+    batch_id = 0
+    opt_steps = p["opt_steps"]
+    for group in group_ligands(ligands):
+        for start in range(0, len(group), MAX_BATCH):
+            idx = group[start:start + MAX_BATCH]
+            batch = [ligands[i] for i in idx]
+            batch_id += 1
+            if opt_steps == -1:  # heuristic of the first batch persists (screening.cpp:206-208)
+                opt_steps = heuristic_opt_steps(max(l.natom for l in batch))
+            prm = DockParams(nchain=p["exhaustiveness"], mc_steps=p["mc_steps"], opt_steps=opt_steps,
+                             refine_steps=p["refine_steps"], num_pose=p["num_pose"],
+                             rmsd_limit=p["rmsd_limit"], energy_range=p["energy_range"], seed=p["seed"],
+                             randomize=p["randomize"], mode=inter_mode, energy_decomp=p["energy_decomp"])
+            t0 = time.time()
+            ls = N.LigSet(batch, bias=p["bias"], bias_k=p["bias_k"], constraint=p["constraint_docking"])
+            results, _ = eng.dock(ls, prm)
+            # gen_filepath, src/myutils/myio.cpp:7-13: an empty directory means the working directory
+            name = f"{p['name_json']}_{batch_id}.json"
+            path = os.path.join(p["output_dir"], name) if p["output_dir"] else name
+            write_json_with_retry(path, dumps_poses(results))
+            files.append(path)
+            log.info("Batch %d (%d ligands) running time: %.1f ms", batch_id, len(batch),
+                     (time.time() - t0) * 1e3)
+    return files
