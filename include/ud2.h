@@ -12,6 +12,7 @@
 #ifndef UD2_H
 #define UD2_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -184,6 +185,52 @@ int ud2_final_score(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* 
  * largest natom among the selected poses. */
 int ud2_score_decomp(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const int32_t* sel_d,
                      int32_t n_sel, int32_t atom_stride, float* decomp_d, void* stream);
+
+/* ------------------------------------------------------------- whole docking run
+ * The reference's core_pipeline (E/src/screening/core.cpp:100-223) behind a
+ * handle: ud2_create takes the run options of CoreInput (E/src/screening/core.h:70-100;
+ * the keyword arguments of DockingPipeline, E/api/python/pipeline.cpp:167-189),
+ * ud2_run_json parses an engine-JSON document (receptor under "receptor", one
+ * member per ligand, optional "score"; E/src/format/json.cpp:34-76 and
+ * E/src/format/rapidjson_parser.cpp:25-296), applies the search-mode and task
+ * presets, docks the ligands in size groups and batches of at most 1,000
+ * (E/src/screening/screening.cpp:140-219) and writes
+ * <out_dir>/<name_json>_<batch>.json (out_dir "" = working directory) in the
+ * reference's output format (E/src/format/json.cpp:88-191). *n_files_h (optional)
+ * = batches written. Host memory only crosses the ABI; the device work runs on
+ * the device gpu_device_id on the default stream. One run per handle at a time. */
+typedef struct ud2_handle ud2_handle;
+
+typedef struct {
+    float center[3], size[3];   /* docking box: lo = center - size/2, hi = center + size/2 */
+    const char* task;           /* screen | score | mc | randomize | optimize (NULL = screen) */
+    const char* search_mode;    /* fast | balance | detail | free (NULL = balance) */
+    int32_t exhaustiveness;     /* MC chains per ligand */
+    int32_t randomize;
+    int32_t mc_steps;
+    int32_t opt_steps;          /* < 0: (25 + natom_max of the first batch) / 3 */
+    int32_t refine_steps;
+    int32_t num_pose;
+    float rmsd_limit;
+    float energy_range;
+    int64_t seed;               /* its low 32 bits key the random streams */
+    const char* bias;           /* no | pos | align (NULL = no) */
+    float bias_k;
+    int32_t constraint_docking;
+    int32_t use_tor_lib;
+    int32_t energy_decomp;
+    int32_t gpu_device_id;
+    int32_t max_gpu_mem;        /* MB; accepted, the batches are bounded by count (288 GB per device) */
+    int32_t inter_mode;         /* UD2_MODE_GRID or UD2_MODE_DIRECT for the search and refinement */
+} ud2_run_params;
+
+/* DockingPipeline's constructor / CoreInput set-up (E/api/python/pipeline.cpp:56-100). */
+int ud2_create(const ud2_run_params* params, ud2_handle** out);
+/* core_pipeline over a JSON text (E/src/screening/core.cpp:100-223; the pybind
+ * run() serialises its objects to this text, E/api/python/pipeline.cpp:120-145). */
+int ud2_run_json(ud2_handle* h, const char* json, size_t len, const char* out_dir, const char* name_json,
+                 int32_t* n_files_h);
+int ud2_destroy(ud2_handle* h);
 
 /* ------------------------------------------------------------- learned proposal
  * No reference counterpart: the reference's MC tries one random mutation per

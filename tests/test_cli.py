@@ -81,28 +81,28 @@ def test_dump_config_template(tmp_path, monkeypatch):
     assert cli.main(["--dump_config", str(tmp_path / "x.yaml")]) == 0
     assert (tmp_path / "x.yaml").read_text() == (tmp_path / "default.yaml").read_text()
     # read back by the CLI's reader, the template's text values are the defaults ("no" stays a string)
-    p, box, fp_json = cli.read_config(str(tmp_path / "x.yaml"))
+    p, fp_json = cli.read_config(str(tmp_path / "x.yaml"))
     assert p["bias"] == "no" and p["bias_k"] == pytest.approx(0.1) and p["randomize"] is True
     assert fp_json == ""
 
 
 def test_read_config_of_the_bias_feature(tmp_path):
-    p, box, fp_json = cli.read_config(write(tmp_path / "ud2.yaml", BIAS_HBOND_YAML))
+    p, fp_json = cli.read_config(write(tmp_path / "ud2.yaml", BIAS_HBOND_YAML))
     assert fp_json == "../../data/bias_hbond/input.json"
     assert p == dict(num_pose=10, rmsd_limit=1.0, energy_range=10.0, seed=1234567, exhaustiveness=512,
                      randomize=True, mc_steps=40, opt_steps=-1, refine_steps=0, use_tor_lib=False, bias="pos",
                      bias_k=1.0, energy_decomp=False, task="screen", search_mode="free", constraint_docking=False,
+                     center_x=2.0, center_y=60.0, center_z=10.0, size_x=30.0, size_y=30.0, size_z=30.0,
                      gpu_device_id=0, max_gpu_mem=0, output_dir="./")
-    assert list(box.lo) == [-13.0, 45.0, -5.0] and list(box.hi) == [17.0, 75.0, 25.0]
 
 
 def test_missing_items_take_defaults(tmp_path, caplog):
     path = write(tmp_path / "c.yaml", "Settings:\n  center_x: 1.5\nInputs:\n  json: a.json\n")
     with caplog.at_level("WARNING", logger="unidock_engine"):
-        p, box, fp_json = cli.read_config(path)
+        p, fp_json = cli.read_config(path)
     assert p["seed"] == 12345 and p["mc_steps"] == 20 and p["search_mode"] == "balance"
     assert p["output_dir"] == "" and fp_json == "a.json"
-    assert list(box.lo) == [1.5, 0.0, 0.0] and list(box.hi) == [1.5, 0.0, 0.0]
+    assert (p["center_x"], p["center_y"], p["size_z"]) == (1.5, 0.0, 0.0)
     assert "Config item Advanced.seed doesn't exist" in caplog.text
 
 

@@ -1,0 +1,106 @@
+"""The whole-run entry point ud2_run_json (the docking pipeline orchestrated inside libud2.so)
+against the same pipeline driven from Python over the kernel-level entry points
+(DockingPipeline.host = "python"). Both make the same kernel calls with the same arguments, so the
+written files must be byte-identical: the topology parsing, presets, size groups, filter, affinity
+and the output writer of the native host are checked against the Python ones on each case."""
+import json
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+BOX_5S8I = dict(center_x=-22.33497980811559, center_y=13.31094327905649, center_z=27.36396790165424,
+                size_x=30.0, size_y=30.0, size_z=30.0)
+BOX_BIAS = dict(center_x=2.0, center_y=60.0, center_z=10.0, size_x=30.0, size_y=30.0, size_z=30.0)
+
+
+def golden(name):
+    doc = json.load(open(os.path.join(HERE, "golden", name)))
+    return doc["receptor"], {k: v for k, v in doc.items() if k not in ("receptor", "score")}
+
+
+def synthetic_doc():
+    """Nine synthetic ligands over two size groups (<= 40 atoms and 8 torsions, and larger)."""
+    from unidock_engine import synthetic as S
+    ligs = S.config1_ligands(24)
+    small = [s for s in ligs if s.ligand.natom <= 40 and s.ligand.ntors <= 8][:5]
+    large = [s for s in ligs if s.ligand.natom > 40 or s.ligand.ntors > 8][:4]
+    assert len(small) == 5 and len(large) == 4
+    c = S.POCKET_CENTER
+    box = dict(center_x=c[0], center_y=c[1], center_z=c[2], size_x=S.BOX_SIZE, size_y=S.BOX_SIZE, size_z=S.BOX_SIZE)
+    return S.receptor_json(S.make_receptor()), {s.ligand.name: s.json_obj for s in large + small}, box
+
+
+CASES = {
+    "5s8i_grid": ("5S8I", dict(search_mode="free", exhaustiveness=64, mc_steps=10, refine_steps=2, seed=121), 1),
+    "5s8i_direct_tor_lib": ("5S8I", dict(search_mode="free", exhaustiveness=32, mc_steps=6, refine_steps=1, seed=5,
+                                         use_tor_lib=True), 0),
+    "bias_pos": ("bias", dict(search_mode="free", exhaustiveness=64, mc_steps=10, refine_steps=0, seed=1234567,
+                              bias="pos", bias_k=1.0), 1),
+    "bias_align_constraint": ("bias", dict(search_mode="free", exhaustiveness=32, mc_steps=6, seed=3, bias="align",
+                                           bias_k=0.5, constraint_docking=True), 1),
+    "synthetic_groups_decomp": ("synthetic", dict(search_mode="fast", seed=11, num_pose=4, energy_decomp=True), 1),
+    "synthetic_direct_detail": ("synthetic", dict(search_mode="detail", exhaustiveness=8, seed=12, rmsd_limit=2.0,
+                                                  energy_range=3.0, refine_steps=3), 0),
+    "task_score": ("5S8I", dict(task="score"), 1),
+    "task_mc": ("synthetic", dict(task="mc", search_mode="free", exhaustiveness=16, mc_steps=8, seed=2), 1),
+    "task_randomize": ("5S8I", dict(task="randomize", search_mode="free", exhaustiveness=16, seed=9), 1),
+    "task_optimize": ("synthetic", dict(task="optimize", seed=4), 0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_native_run_matches_python_driven_run(gpu, tmp_path, case):
+    from unidock_engine.api.python import pipeline
+    src, kw, mode = CASES[case]
+    if src == "5S8I":
+        rec, ligs = golden("5S8I_unidock2.json")
+        box = BOX_5S8I
+    elif src == "bias":
+        rec, ligs = golden("bias_hbond_input.json")
+        box = BOX_BIAS
+    else:
+        rec, ligs, box = synthetic_doc()
+    print(f"host parity case {case}", flush=True)
+    outs = {}
+    for run, host in (("native", "native"), ("native2", "native"), ("python", "python")):
+        d = tmp_path / run
+        p = pipeline.DockingPipeline(output_dir=str(d), name_json="run", **box, **kw)
+        p.inter_mode = mode
+        p.host = host
+        p.set_receptor(rec)
+        p.add_ligands(ligs)
+        p.run()
+        assert p.output_files, run
+        outs[run] = {os.path.basename(f): open(f, "rb").read() for f in p.output_files}
+    assert sorted(os.listdir(tmp_path / "native")) == sorted(outs["native"])
+    assert outs["native"] == outs["native2"], "two native runs differ"
+    assert list(outs["native"]) == list(outs["python"])
+    for name in outs["native"]:
+        a, b = outs["native"][name], outs["python"][name]
+        i = next((k for k in range(min(len(a), len(b))) if a[k] != b[k]), None)
+        assert a == b, (name, len(a), len(b), i, a[max(0, (i or 0) - 120):(i or 0) + 80])
+    doc = json.loads(next(iter(outs["native"].values())))
+    assert doc and all(isinstance(v, list) for v in doc.values())
+    if kw.get("energy_decomp"):
+        assert all("decomp" in pose for poses in doc.values() for pose in poses)
+
+
+def test_native_run_failures(gpu, tmp_path):
+    """Bad options and bad input become RuntimeError("Core pipeline failed.") with the reason in the
+    log, as the reference's pipeline reports a failed core_pipeline (pipeline.cpp:141-143)."""
+    from unidock_engine.api.python import pipeline
+    rec, ligs = golden("5S8I_unidock2.json")
+    for kw, data in ((dict(task="dock"), ligs), (dict(search_mode="turbo"), ligs),
+                     (dict(), {"bad": {"atoms": [[0.0, 1.0]]}})):
+        p = pipeline.DockingPipeline(output_dir=str(tmp_path), **BOX_5S8I, **kw)
+        p.set_receptor(rec)
+        p.add_ligands(data)
+        with pytest.raises(RuntimeError, match="Core pipeline failed."):
+            p.run()
+    # no ligands: nothing to dock, nothing written, not an error (json.cpp:71-73)
+    p = pipeline.DockingPipeline(output_dir=str(tmp_path / "empty"), **BOX_5S8I)
+    p.set_receptor(rec)
+    p.run()
+    assert p.output_files == []

@@ -102,3 +102,34 @@ def test_decomp_member_in_output_document():
     pose = doc["m"][0]
     assert list(pose) == ["energy", "coords", "dihedrals", "decomp"]
     assert pose["decomp"] == [[-0.5, -0.25, 0.125, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 0.0]]
+
+
+def _native_params(**kw):
+    p = dict(P.DEFAULTS, center_x=0.0, center_y=0.0, center_z=0.0, size_x=30.0, size_y=30.0, size_z=30.0)
+    p.update(kw)
+    return p
+
+
+def test_native_run_rejects_bad_options_and_input(tmp_path):
+    """ud2_run_json's checks that come before any device work: options, JSON syntax, topology."""
+    from unidock_engine import _native as N
+    out = str(tmp_path)
+    good_atom = [0.0, 0.0, 0.0, 2, 3, 0.0, [], []]
+    cases = [
+        (dict(bias="turbo"), "{}", "Not supported bias: turbo"),
+        (dict(search_mode="warp"), "{}", "Not supported search_mode: warp"),
+        (dict(task="dock"), "{}", "Not supported task: dock"),
+        (dict(), '{"a": [1, 2,]}', "JSON parse error at offset 12: unexpected character"),
+        (dict(), '{"a": 1} x', "trailing characters"),
+        (dict(), '{"lig": {"atoms": [[0.0, 1.0]]}}', "missing array element"),
+        (dict(), json.dumps({"lig": {"torsions": []}}), "ligand lig: no atoms"),
+        (dict(), json.dumps({"lig": {"atoms": [good_atom + [[[1.0, 2.0]]]]}}), "bias site of atom 0"),
+        (dict(use_tor_lib=True), json.dumps({"lig": {"atoms": [good_atom] * 4, "torsions": [
+            [[0, 1, 2, 3], 10.0, [[-30.0, -60.0]], [2, 3]]]}}), "wrong torsion range"),
+    ]
+    for kw, text, msg in cases:
+        with pytest.raises(N.Ud2Error, match=msg):
+            N.run_json(_native_params(output_dir=out, name_json="x", **kw), text)
+    # no ligands: nothing written, no error
+    assert N.run_json(_native_params(output_dir=out, name_json="x"), '{"receptor": [], "score": ["vina"]}') == 0
+    assert not list(tmp_path.iterdir())

@@ -64,6 +64,10 @@ int odd_ld(int dim) { return (dim % 2 == 1) ? dim : dim + 1; }
 
 }  // namespace
 
+namespace ud2 {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ud2
+
 struct ud2_scene {
     int device = 0;
     float4* d_rec = nullptr;

@@ -1,0 +1,970 @@
+// Native host orchestration of a docking run: ud2_create / ud2_run_json / ud2_destroy.
+//
+// One ud2_run_json call is the reference's core_pipeline over an engine-JSON
+// document (E/src/screening/core.cpp:100-223 with E/src/format/json.cpp:34-76
+// and E/src/screening/screening.cpp:140-219): parse the document, build the
+// receptor and the ligand topologies (E/src/format/rapidjson_parser.cpp:25-296),
+// apply the search-mode and task presets, split the ligands into size groups
+// and batches of at most 1,000, dock each batch through the kernel-level entry
+// points of this library (MC search, clustering, refinement with the penalty
+// ramp, filter, final score; E/src/task/DockTask.cpp:39-282) and write
+// <out_dir>/<name_json>_<batch>.json in the reference's output format
+// (E/src/format/json.cpp:88-191). Host arithmetic that reaches the output is
+// FP32 in the reference's order of operations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <charconv>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <map>
+#include <set>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include <unistd.h>
+
+#include "../../include/ud2.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace ud2 {
+int set_error(int code, const std::string& msg);  // ud2_api.cpp: the calling thread's last error
+}
+
+namespace {
+
+constexpr float kPi = 3.1415927f;                      // PI of E/src/myutils/common.h:45
+constexpr float kDeg2Rad = kPi / 180.0f;
+constexpr float kRad2Deg = 180.0f / kPi;
+constexpr float kVinaCutoff = 8.0f;                    // E/src/constants/constants.h:9
+constexpr float kWConfIndep = 0.05846f;                // E/src/score/vina.h:123
+constexpr float kVCapAuthentic = 1000.0f;              // E/src/constants/constants.h:19
+constexpr float kPenaltySlope = 1e6f;                  // E/src/constants/constants.h:16
+constexpr double kMaxSafe = 1e6;                       // E/src/format/json.cpp:78-85
+constexpr int kMaxBatch = 1000;                        // E/src/screening/screening.cpp:183
+constexpr int kGroups[4][2] = {{40, 8}, {80, 16}, {120, 24}, {160, 32}};  // screening.h:19-26
+constexpr int32_t kDefaultProbes[8] = {2, 3, 4, 5, 6, 10, 11, 15};
+constexpr float kGridSpacing = 0.375f;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void raise(const std::string& msg, int code = UD2_ERR_ARG) { throw Error{code, msg}; }
+
+void check(int rc, const char* what) {
+    if (rc != UD2_OK) throw Error{rc, std::string(what) + ": " + ud2_last_error()};
+}
+
+// ------------------------------------------------------------------ JSON
+struct JVal {
+    enum Kind : uint8_t { Null, Bool, Num, Str, Arr, Obj } kind = Null;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;  // document order, duplicates kept
+
+    const JVal* find(const char* key) const {  // first member of that name
+        for (auto& kv : obj)
+            if (kv.first == key) return &kv.second;
+        return nullptr;
+    }
+};
+
+class Parser {
+  public:
+    Parser(const char* s, size_t n) : p_(s), end_(s + n) {}
+
+    JVal document() {
+        JVal v = value(0);
+        ws();
+        if (p_ != end_) fail("trailing characters");
+        return v;
+    }
+
+  private:
+    const char* p_;
+    const char* end_;
+    const char* begin_ = p_;
+
+    [[noreturn]] void fail(const char* what) {
+        raise(std::string("JSON parse error at offset ") + std::to_string(p_ - begin_) + ": " + what);
+    }
+    void ws() {
+        while (p_ < end_ && (*p_ == ' ' || *p_ == '\t' || *p_ == '\n' || *p_ == '\r')) ++p_;
+    }
+    bool lit(const char* s) {
+        size_t n = std::strlen(s);
+        if (size_t(end_ - p_) >= n && std::memcmp(p_, s, n) == 0) {
+            p_ += n;
+            return true;
+        }
+        return false;
+    }
+    JVal value(int depth) {
+        if (depth > 512) fail("nesting too deep");
+        ws();
+        if (p_ >= end_) fail("unexpected end");
+        JVal v;
+        char c = *p_;
+        if (c == '{') {
+            v.kind = JVal::Obj;
+            ++p_;
+            ws();
+            if (p_ < end_ && *p_ == '}') {
+                ++p_;
+                return v;
+            }
+            for (;;) {
+                ws();
+                if (p_ >= end_ || *p_ != '"') fail("expected a member name");
+                std::string key = string();
+                ws();
+                if (p_ >= end_ || *p_ != ':') fail("expected ':'");
+                ++p_;
+                v.obj.emplace_back(std::move(key), value(depth + 1));
+                ws();
+                if (p_ < end_ && *p_ == ',') {
+                    ++p_;
+                    continue;
+                }
+                if (p_ < end_ && *p_ == '}') {
+                    ++p_;
+                    return v;
+                }
+                fail("expected ',' or '}'");
+            }
+        }
+        if (c == '[') {
+            v.kind = JVal::Arr;
+            ++p_;
+            ws();
+            if (p_ < end_ && *p_ == ']') {
+                ++p_;
+                return v;
+            }
+            for (;;) {
+                v.arr.push_back(value(depth + 1));
+                ws();
+                if (p_ < end_ && *p_ == ',') {
+                    ++p_;
+                    continue;
+                }
+                if (p_ < end_ && *p_ == ']') {
+                    ++p_;
+                    return v;
+                }
+                fail("expected ',' or ']'");
+            }
+        }
+        if (c == '"') {
+            v.kind = JVal::Str;
+            v.str = string();
+            return v;
+        }
+        if (lit("true")) {
+            v.kind = JVal::Bool;
+            v.b = true;
+            return v;
+        }
+        if (lit("false")) {
+            v.kind = JVal::Bool;
+            return v;
+        }
+        if (lit("null")) return v;
+        if (c == '-' || (c >= '0' && c <= '9')) {
+            const char* q = p_;
+            if (*q == '-') ++q;
+            while (q < end_ && ((*q >= '0' && *q <= '9') || *q == '.' || *q == 'e' || *q == 'E' || *q == '+' ||
+                                *q == '-'))
+                ++q;
+            auto r = std::from_chars(p_, q, v.num);
+            if (r.ec != std::errc() || r.ptr != q) fail("bad number");
+            v.kind = JVal::Num;
+            p_ = q;
+            return v;
+        }
+        fail("unexpected character");
+    }
+    static void utf8(std::string& out, uint32_t cp) {
+        if (cp < 0x80) {
+            out += char(cp);
+        } else if (cp < 0x800) {
+            out += char(0xC0 | (cp >> 6));
+            out += char(0x80 | (cp & 0x3F));
+        } else if (cp < 0x10000) {
+            out += char(0xE0 | (cp >> 12));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        } else {
+            out += char(0xF0 | (cp >> 18));
+            out += char(0x80 | ((cp >> 12) & 0x3F));
+            out += char(0x80 | ((cp >> 6) & 0x3F));
+            out += char(0x80 | (cp & 0x3F));
+        }
+    }
+    uint32_t hex4() {
+        if (end_ - p_ < 4) fail("short \\u escape");
+        uint32_t v = 0;
+        for (int i = 0; i < 4; i++) {
+            char c = *p_++;
+            v <<= 4;
+            if (c >= '0' && c <= '9') v |= uint32_t(c - '0');
+            else if (c >= 'a' && c <= 'f') v |= uint32_t(c - 'a' + 10);
+            else if (c >= 'A' && c <= 'F') v |= uint32_t(c - 'A' + 10);
+            else fail("bad \\u escape");
+        }
+        return v;
+    }
+    std::string string() {
+        ++p_;  // opening quote
+        std::string out;
+        while (p_ < end_ && *p_ != '"') {
+            char c = *p_++;
+            if (static_cast<unsigned char>(c) < 0x20) fail("control character in string");
+            if (c != '\\') {
+                out += c;
+                continue;
+            }
+            if (p_ >= end_) fail("unterminated escape");
+            char e = *p_++;
+            switch (e) {
+                case '"': out += '"'; break;
+                case '\\': out += '\\'; break;
+                case '/': out += '/'; break;
+                case 'b': out += '\b'; break;
+                case 'f': out += '\f'; break;
+                case 'n': out += '\n'; break;
+                case 'r': out += '\r'; break;
+                case 't': out += '\t'; break;
+                case 'u': {
+                    uint32_t cp = hex4();
+                    if (cp >= 0xD800 && cp < 0xDC00 && end_ - p_ >= 6 && p_[0] == '\\' && p_[1] == 'u') {
+                        p_ += 2;
+                        uint32_t lo = hex4();
+                        if (lo < 0xDC00 || lo >= 0xE000) fail("bad surrogate pair");
+                        cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                    }
+                    utf8(out, cp);
+                    break;
+                }
+                default: fail("bad escape");
+            }
+        }
+        if (p_ >= end_) fail("unterminated string");
+        ++p_;
+        return out;
+    }
+};
+
+const JVal& arr_at(const JVal& v, size_t i, const char* what) {
+    if (v.kind != JVal::Arr || i >= v.arr.size()) raise(std::string(what) + ": missing array element");
+    return v.arr[i];
+}
+
+double num(const JVal& v, const char* what) {
+    if (v.kind != JVal::Num) raise(std::string(what) + ": expected a number");
+    return v.num;
+}
+
+int inum(const JVal& v, const char* what) {
+    double d = num(v, what);
+    if (!(d > -2147483648.0 && d < 2147483648.0)) raise(std::string(what) + ": integer out of range");
+    return int(d);  // truncation, as int() of a JSON number
+}
+
+const JVal& arr(const JVal& v, const char* what) {
+    if (v.kind != JVal::Arr) raise(std::string(what) + ": expected an array");
+    return v;
+}
+
+// ------------------------------------------------------------------ topology
+struct Lig {
+    std::string name;
+    int natom = 0, ntors = 0;
+    std::vector<float> xyz;
+    std::vector<int32_t> types;
+    std::vector<float> dihe;
+    std::vector<int32_t> axis;  // [ntors][2]
+    std::vector<std::vector<int32_t>> rot;
+    std::vector<std::vector<std::pair<float, float>>> ranges;
+    std::vector<int32_t> pairs;  // [npair][2]
+    std::vector<int32_t> bias_count;  // [natom]
+    std::vector<float> bias;          // [sites][5]
+};
+
+// Allowed dihedral intervals in radians (rapidjson_parser.cpp:151-173).
+std::vector<std::pair<float, float>> torsion_ranges(const JVal& rd, bool use_tor_lib) {
+    if (!use_tor_lib) return {{-kPi, kPi}};
+    std::vector<std::pair<float, float>> out;
+    for (auto& r : arr(rd, "torsion ranges").arr) {
+        float lo_d = float(num(arr_at(r, 0, "range"), "range")), hi_d = float(num(arr_at(r, 1, "range"), "range"));
+        float lo = lo_d * kDeg2Rad, hi = hi_d * kDeg2Rad;
+        if (lo < hi) {
+            out.push_back({lo, hi});
+        } else if (lo > 0 && hi < 0) {
+            out.push_back({lo, kPi});
+            out.push_back({-kPi, hi});
+        } else {
+            raise("wrong torsion range, lo > hi: " + std::to_string(lo_d) + ", " + std::to_string(hi_d));
+        }
+    }
+    return out;
+}
+
+// Rigid fragments (RapidJsonParser::split_torsions_into_frags, rapidjson_parser.cpp:227-296):
+// torsions touching the root make their own fragment (rotated atoms + axis) and add their axis
+// to the root fragment; every other torsion splits the first fragment holding both axis atoms
+// into (fragment & rotated) and (fragment - rotated), both with the axis added.
+std::vector<std::set<int>> split_frags(const std::set<int>& root, const Lig& L) {
+    std::set<int> frag0 = root;
+    std::vector<std::set<int>> frags;
+    std::vector<int> remaining;
+    for (int t = 0; t < L.ntors; t++) {
+        int a0 = L.axis[2 * t], a1 = L.axis[2 * t + 1];
+        if (root.count(a0) || root.count(a1)) {
+            frag0.insert(a0);
+            frag0.insert(a1);
+            std::set<int> f(L.rot[t].begin(), L.rot[t].end());
+            f.insert(a0);
+            f.insert(a1);
+            frags.push_back(std::move(f));
+        } else {
+            remaining.push_back(t);
+        }
+    }
+    frags.push_back(frag0);
+    for (int t : remaining) {
+        int a0 = L.axis[2 * t], a1 = L.axis[2 * t + 1];
+        std::set<int> rs(L.rot[t].begin(), L.rot[t].end());
+        for (size_t k = 0; k < frags.size(); k++) {
+            if (!frags[k].count(a0) || !frags[k].count(a1)) continue;
+            std::set<int> f1, f2;
+            for (int a : frags[k]) (rs.count(a) ? f1 : f2).insert(a);
+            f1.insert(a0);
+            f1.insert(a1);
+            f2.insert(a0);
+            f2.insert(a1);
+            frags[k] = std::move(f1);
+            frags.push_back(std::move(f2));
+            break;
+        }
+    }
+    return frags;
+}
+
+// One ligand of the engine JSON (rapidjson_parser.cpp:54-225).
+Lig ligand_from_json(const std::string& name, const JVal& obj, bool use_tor_lib) {
+    Lig L;
+    L.name = name;
+    if (obj.kind != JVal::Obj) raise("ligand " + name + ": expected an object");
+    const JVal* atoms = obj.find("atoms");
+    if (!atoms) raise("ligand " + name + ": no atoms");
+    const int n = int(arr(*atoms, "atoms").arr.size());
+    L.natom = n;
+    L.xyz.resize(3 * size_t(n));
+    L.types.resize(n);
+    L.bias_count.assign(n, 0);
+    std::set<std::pair<int, int>> p1213, p14;
+    for (int ia = 0; ia < n; ia++) {
+        const JVal& a = atoms->arr[ia];
+        for (int k = 0; k < 3; k++) L.xyz[3 * ia + k] = float(num(arr_at(a, k, "atom"), "atom coordinate"));
+        L.types[ia] = inum(arr_at(a, 3, "atom"), "atom type");
+        num(arr_at(a, 4, "atom"), "atom ff type");
+        num(arr_at(a, 5, "atom"), "atom charge");
+        for (auto& b : arr(arr_at(a, 6, "atom"), "1-2/1-3 list").arr) {
+            int ib = inum(b, "1-2/1-3 index");
+            if (ib != ia) p1213.insert({std::min(ia, ib), std::max(ia, ib)});
+        }
+    }
+    for (int ia = 0; ia < n; ia++) {
+        for (auto& b : arr(arr_at(atoms->arr[ia], 7, "atom"), "1-4 list").arr) {
+            int ib = inum(b, "1-4 index");
+            if (ib == ia) continue;
+            std::pair<int, int> p{std::min(ia, ib), std::max(ia, ib)};
+            if (!p1213.count(p)) p14.insert(p);
+        }
+    }
+    // bias sites, the ninth field of an atom line (rapidjson_parser.cpp:113-124)
+    for (int ia = 0; ia < n; ia++) {
+        const JVal& a = atoms->arr[ia];
+        if (a.arr.size() <= 8) continue;
+        for (auto& b : arr(a.arr[8], "bias sites").arr) {
+            if (b.kind != JVal::Arr || b.arr.size() < 5)
+                raise("bias site of atom " + std::to_string(ia) + " has fewer than 5 values");
+            for (int k = 0; k < 5; k++) L.bias.push_back(float(num(b.arr[k], "bias site")));
+            L.bias_count[ia]++;
+        }
+    }
+    if (const JVal* tors = obj.find("torsions")) {
+        for (auto& tor : arr(*tors, "torsions").arr) {
+            const JVal& four = arr_at(tor, 0, "torsion");
+            L.axis.push_back(inum(arr_at(four, 1, "torsion atoms"), "torsion atom"));
+            L.axis.push_back(inum(arr_at(four, 2, "torsion atoms"), "torsion atom"));
+            L.dihe.push_back(float(num(arr_at(tor, 1, "torsion"), "dihedral")) * kDeg2Rad);
+            L.ranges.push_back(torsion_ranges(arr_at(tor, 2, "torsion"), use_tor_lib));
+            std::vector<int32_t> r;
+            for (auto& a : arr(arr_at(tor, 3, "torsion"), "rotated atoms").arr) r.push_back(inum(a, "rotated atom"));
+            L.rot.push_back(std::move(r));
+        }
+    }
+    L.ntors = int(L.dihe.size());
+    std::set<int> root;
+    if (const JVal* ra = obj.find("root_atoms"))
+        for (auto& a : arr(*ra, "root_atoms").arr) root.insert(inum(a, "root atom"));
+    std::vector<std::set<int>> frags = split_frags(root, L);
+    // scored intra pairs i < j (rapidjson_parser.cpp:201-221)
+    std::vector<std::vector<int>> frag_of(n);
+    for (size_t k = 0; k < frags.size(); k++)
+        for (int a : frags[k])
+            if (a >= 0 && a < n) frag_of[a].push_back(int(k));
+    for (int i = 0; i < n; i++) {
+        for (int j = i + 1; j < n; j++) {
+            if (p1213.count({i, j}) || p14.count({i, j})) continue;
+            bool shared = false;
+            for (int fi : frag_of[i]) {
+                if (std::find(frag_of[j].begin(), frag_of[j].end(), fi) != frag_of[j].end()) {
+                    shared = true;
+                    break;
+                }
+            }
+            if (shared) continue;
+            L.pairs.push_back(i);
+            L.pairs.push_back(j);
+        }
+    }
+    return L;
+}
+
+struct Rec {
+    std::vector<float> xyz;
+    std::vector<int32_t> types;
+};
+
+// Receptor atoms [x, y, z, xs, ff, q]: hydrogens dropped, cropped to the box grown by the
+// cutoff (rapidjson_parser.cpp:25-52, json.cpp:58-65).
+Rec receptor_from_json(const JVal* atoms, const float lo[3], const float hi[3]) {
+    Rec R;
+    if (!atoms) return R;
+    float glo[3], ghi[3];
+    for (int d = 0; d < 3; d++) {
+        glo[d] = lo[d] - kVinaCutoff;
+        ghi[d] = hi[d] + kVinaCutoff;
+    }
+    for (auto& a : arr(*atoms, "receptor").arr) {
+        int t = inum(arr_at(a, 3, "receptor atom"), "receptor atom type");
+        if (t == 0) continue;  // XS_TYPE_H
+        float x[3];
+        bool in = true;
+        for (int d = 0; d < 3; d++) {
+            x[d] = float(num(arr_at(a, d, "receptor atom"), "receptor coordinate"));
+            in = in && x[d] >= glo[d] && x[d] <= ghi[d];
+        }
+        num(arr_at(a, 4, "receptor atom"), "receptor ff type");
+        num(arr_at(a, 5, "receptor atom"), "receptor charge");
+        if (!in) continue;
+        R.xyz.insert(R.xyz.end(), x, x + 3);
+        R.types.push_back(t);
+    }
+    return R;
+}
+
+// ------------------------------------------------------------------ run parameters
+struct Params {
+    float lo[3], hi[3];
+    std::string task, search_mode, bias;
+    int exhaustiveness, mc_steps, opt_steps, refine_steps, num_pose;
+    bool randomize, constraint, use_tor_lib, energy_decomp;
+    float rmsd_limit, energy_range, bias_k;
+    int64_t seed;
+    int gpu, inter_mode;
+};
+
+// apply_search_mode, core.cpp:65-84
+void apply_search_mode(Params& p) {
+    const std::string& m = p.search_mode;
+    if (m.empty() || m == "free") return;
+    if (m == "fast") p.exhaustiveness = 128, p.mc_steps = 20;
+    else if (m == "balance") p.exhaustiveness = 256, p.mc_steps = 30;
+    else if (m == "detail") p.exhaustiveness = 512, p.mc_steps = 40;
+    else raise("Not supported search_mode: " + m + " doesn't belong to (fast, balance, detail, free)");
+    p.opt_steps = -1;
+}
+
+// task presets of core_pipeline, core.cpp:165-209
+void apply_task(Params& p) {
+    const std::string& t = p.task;
+    if (t == "screen") return;
+    if (t == "score") {
+        p.randomize = false, p.exhaustiveness = 1, p.mc_steps = 0, p.opt_steps = 0, p.refine_steps = 0;
+        p.num_pose = 1, p.energy_range = 999.0f, p.rmsd_limit = 999.0f;
+    } else if (t == "mc") {
+        p.randomize = true, p.opt_steps = 0, p.refine_steps = 0;
+    } else if (t == "randomize") {
+        p.randomize = true, p.mc_steps = 0, p.opt_steps = 0, p.refine_steps = 0, p.num_pose = p.exhaustiveness;
+        p.energy_range = 1e9f, p.rmsd_limit = 0.0f;
+    } else if (t == "optimize") {
+        p.randomize = false, p.exhaustiveness = 1, p.mc_steps = 0, p.opt_steps = 0, p.num_pose = 1;
+        p.energy_range = 1e9f, p.rmsd_limit = 0.0f;
+    } else {
+        raise("Not supported task: " + t + " doesn't belong to (screen, score, mc, randomize, optimize)");
+    }
+}
+
+// ------------------------------------------------------------------ output
+// A float as the reference's writer prints it: the shortest round-trip digits of its double
+// value, at most 3 decimals (extra digits dropped, not rounded), at least one.
+void put_float(std::string& out, float f) {
+    double v = f;
+    if (std::isnan(v) || std::isinf(v)) v = kMaxSafe;
+    v = std::min(std::max(v, -kMaxSafe), kMaxSafe);
+    char buf[64];
+    auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::fixed);
+    std::string s(buf, r.ptr);
+    size_t dot = s.find('.');
+    if (dot == std::string::npos) {
+        out += s;
+        out += ".0";
+        return;
+    }
+    std::string fp = s.substr(dot + 1, 3);
+    while (!fp.empty() && fp.back() == '0') fp.pop_back();
+    out.append(s, 0, dot + 1);
+    out += fp.empty() ? "0" : fp;
+}
+
+void put_string(std::string& out, const std::string& s) {
+    out += '"';
+    for (unsigned char c : s) {
+        switch (c) {
+            case '"': out += "\\\""; break;
+            case '\\': out += "\\\\"; break;
+            case '\b': out += "\\b"; break;
+            case '\f': out += "\\f"; break;
+            case '\n': out += "\\n"; break;
+            case '\r': out += "\\r"; break;
+            case '\t': out += "\\t"; break;
+            default:
+                if (c < 0x20) {
+                    char b[8];
+                    std::snprintf(b, sizeof b, "\\u%04X", c);
+                    out += b;
+                } else {
+                    out += char(c);
+                }
+        }
+    }
+    out += '"';
+}
+
+// dump_poses, E/src/task/DockTask.cpp:285-334: write, flush to disk, check that the file ends
+// with '}', retry with doubling back-off.
+void write_with_retry(const std::string& path, const std::string& text) {
+    double wait = 5.0;
+    for (int attempt = 1;; attempt++) {
+        bool ok = false;
+        if (FILE* f = std::fopen(path.c_str(), "wb")) {
+            ok = std::fwrite(text.data(), 1, text.size(), f) == text.size() && std::fflush(f) == 0 &&
+                 fsync(fileno(f)) == 0;
+            ok = (std::fclose(f) == 0) && ok;
+        }
+        if (ok) {
+            if (FILE* f = std::fopen(path.c_str(), "rb")) {
+                ok = std::fseek(f, -1, SEEK_END) == 0 && std::fgetc(f) == '}';
+                std::fclose(f);
+            } else {
+                ok = false;
+            }
+        }
+        if (ok) return;
+        if (attempt == 5) raise("failed to write " + path, UD2_ERR_STATE);
+        std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+        wait *= 2;
+    }
+}
+
+// ------------------------------------------------------------------ device helpers
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    explicit DevBuf(size_t n) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) raise(std::string("hipMalloc: ") + hipGetErrorString(e), UD2_ERR_HIP);
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void zero(size_t n) {
+        hipError_t e = hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) raise(std::string("hipMemset: ") + hipGetErrorString(e), UD2_ERR_HIP);
+    }
+    void to_host(T* h, size_t n) const {
+        if (n == 0) return;
+        hipError_t e = hipMemcpy(h, p, n * sizeof(T), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) raise(std::string("hipMemcpy: ") + hipGetErrorString(e), UD2_ERR_HIP);
+    }
+    void from_host(const T* h, size_t n) {
+        if (n == 0) return;
+        hipError_t e = hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) raise(std::string("hipMemcpy: ") + hipGetErrorString(e), UD2_ERR_HIP);
+    }
+};
+
+struct SceneH {
+    ud2_scene* h = nullptr;
+    ~SceneH() {
+        if (h) ud2_scene_destroy(h);
+    }
+};
+struct LigSetH {
+    ud2_ligset* h = nullptr;
+    ~LigSetH() {
+        if (h) ud2_ligset_destroy(h);
+    }
+};
+
+// Penalty slope per refinement round, 100 * 10^(2p) capped at 1e6 (DockTask.cpp:171-175).
+float refine_slope(int p) { return float(std::min(100.0 * std::pow(10.0, 2.0 * p), double(kPenaltySlope))); }
+
+// DockTask::run for one batch (DockTask.cpp:39-282); returns the batch's output JSON text.
+std::string dock_batch(ud2_scene* sc, const std::vector<const Lig*>& ligs, const Params& p, int opt_steps) {
+    const int n_lig = int(ligs.size());
+    const int nchain = p.exhaustiveness;
+    if (nchain <= 0) raise("exhaustiveness must be positive");
+    // ligand batch in the CSR layout of ud2_ligand_batch
+    std::vector<int32_t> atom_off(n_lig + 1, 0), tors_off(n_lig + 1, 0), pair_off(n_lig + 1, 0);
+    std::vector<int32_t> types, axis, rot_off{0}, rot_atoms, range_off{0}, pairs;
+    std::vector<float> xyz, dihe0, ranges;
+    std::vector<int32_t> bias_off{0};
+    std::vector<float> bias;
+    int max_natom = 0;
+    for (int i = 0; i < n_lig; i++) {
+        const Lig& L = *ligs[i];
+        atom_off[i + 1] = atom_off[i] + L.natom;
+        tors_off[i + 1] = tors_off[i] + L.ntors;
+        pair_off[i + 1] = pair_off[i] + int(L.pairs.size() / 2);
+        max_natom = std::max(max_natom, L.natom);
+        types.insert(types.end(), L.types.begin(), L.types.end());
+        xyz.insert(xyz.end(), L.xyz.begin(), L.xyz.end());
+        axis.insert(axis.end(), L.axis.begin(), L.axis.end());
+        dihe0.insert(dihe0.end(), L.dihe.begin(), L.dihe.end());
+        for (int t = 0; t < L.ntors; t++) {
+            rot_atoms.insert(rot_atoms.end(), L.rot[t].begin(), L.rot[t].end());
+            rot_off.push_back(int32_t(rot_atoms.size()));
+            for (auto& q : L.ranges[t]) {
+                ranges.push_back(q.first);
+                ranges.push_back(q.second);
+            }
+            range_off.push_back(int32_t(ranges.size() / 2));
+        }
+        pairs.insert(pairs.end(), L.pairs.begin(), L.pairs.end());
+        for (int a = 0; a < L.natom; a++) bias_off.push_back(bias_off.back() + L.bias_count[a]);
+        bias.insert(bias.end(), L.bias.begin(), L.bias.end());
+    }
+    int32_t zero_i = 0;
+    float zero_f = 0.0f;
+    ud2_ligand_batch b;
+    b.n_lig = n_lig;
+    b.atom_off_h = atom_off.data();
+    b.tors_off_h = tors_off.data();
+    b.pair_off_h = pair_off.data();
+    b.types_h = types.data();
+    b.xyz_h = xyz.data();
+    b.axis_h = axis.empty() ? &zero_i : axis.data();
+    b.dihe0_h = dihe0.empty() ? &zero_f : dihe0.data();
+    b.rot_off_h = rot_off.data();
+    b.rot_atoms_h = rot_atoms.empty() ? &zero_i : rot_atoms.data();
+    b.range_off_h = range_off.data();
+    b.ranges_h = ranges.empty() ? &zero_f : ranges.data();
+    b.pairs_h = pairs.empty() ? &zero_i : pairs.data();
+    LigSetH ls;
+    check(ud2_ligset_create(&b, &ls.h), "ud2_ligset_create");
+    const int bias_type = p.bias == "pos" ? UD2_BIAS_POS : p.bias == "align" ? UD2_BIAS_ALIGN : UD2_BIAS_NO;
+    if (bias_type != UD2_BIAS_NO) {
+        if (bias.empty()) bias.assign(5, 0.0f);
+        check(ud2_ligset_set_bias(ls.h, bias_type, p.bias_k, bias_off.data(), bias.data()), "ud2_ligset_set_bias");
+    }
+    if (p.constraint) check(ud2_ligset_set_constraint(ls.h, 1), "ud2_ligset_set_constraint");
+
+    int64_t xf = 0, df = 0, npose = 0;
+    check(ud2_ligset_pose_layout(ls.h, nchain, &xf, &df, &npose), "ud2_ligset_pose_layout");
+    DevBuf<float> x(xf), c(3 * npose), rv(3 * npose), dh(df), e(npose);
+    x.zero(xf), c.zero(3 * npose), rv.zero(3 * npose), dh.zero(df), e.zero(npose);
+    ud2_poses P{x.p, c.p, rv.p, dh.p, e.p, nchain};
+    check(ud2_init_poses(ls.h, &P, nullptr), "ud2_init_poses");
+    ud2_mc_params mc{};
+    mc.mc_steps = p.mc_steps;
+    mc.opt_steps = opt_steps;
+    mc.randomize = p.randomize ? 1 : 0;
+    mc.seed = uint32_t(uint64_t(p.seed) & 0xFFFFFFFFu);
+    mc.mode = p.inter_mode;
+    mc.lig_index_base = 0;
+    mc.stats_d = nullptr;
+    check(ud2_mc(sc, ls.h, &P, &mc, nullptr), "ud2_mc");
+
+    // clustering (cluster_cu): leaders per ligand in selection order
+    DevBuf<int32_t> leaders(size_t(n_lig) * nchain), nlead(n_lig);
+    leaders.zero(size_t(n_lig) * nchain), nlead.zero(n_lig);
+    check(ud2_cluster(ls.h, &P, p.rmsd_limit, leaders.p, nlead.p, nullptr), "ud2_cluster");
+    std::vector<int32_t> lead_h(size_t(n_lig) * nchain), nl_h(n_lig);
+    leaders.to_host(lead_h.data(), lead_h.size());
+    nlead.to_host(nl_h.data(), nl_h.size());
+    std::vector<int32_t> sel;
+    for (int l = 0; l < n_lig; l++)
+        for (int k = 0; k < nl_h[l]; k++) sel.push_back(lead_h[size_t(l) * nchain + k] + l * nchain);
+    const int n_sel = int(sel.size());
+    DevBuf<int32_t> sel_d(n_sel);
+    sel_d.from_host(sel.data(), sel.size());
+    // refinement with the penalty-slope ramp (run_refine, DockTask.cpp:155-186)
+    for (int r = 0; r < p.refine_steps && n_sel > 0; r++)
+        check(ud2_optimize(sc, ls.h, &P, sel_d.p, n_sel, p.inter_mode, 2, kVCapAuthentic, refine_slope(r), nullptr,
+                           nullptr),
+              "ud2_optimize");
+    // filter (run_filter, DockTask.cpp:188-220): per ligand, leaders by energy within energy_range
+    std::vector<float> e_h(npose);
+    check(hipDeviceSynchronize() == hipSuccess ? UD2_OK : UD2_ERR_HIP, "hipDeviceSynchronize");
+    e.to_host(e_h.data(), e_h.size());
+    std::vector<int32_t> keep, owner;
+    int pos = 0;
+    for (int l = 0; l < n_lig; l++) {
+        const int n = nl_h[l];
+        std::vector<int> order(n);
+        for (int j = 0; j < n; j++) order[j] = j;
+        auto es = [&](int j) { return e_h[sel[pos + j]]; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            float ea = es(a), eb = es(b);
+            return ea < eb || (!std::isnan(ea) && std::isnan(eb));
+        });
+        if (n > 0) {
+            const float emax = es(order[0]) + p.energy_range;
+            int cnt = 0;
+            for (int j : order) {
+                if (es(j) > emax) break;
+                keep.push_back(sel[pos + j]);
+                owner.push_back(l);
+                if (++cnt >= p.num_pose) break;
+            }
+        }
+        pos += n;
+    }
+    const int n_keep = int(keep.size());
+    std::vector<float> sc4(4 * size_t(n_keep)), dc;
+    if (n_keep > 0) {
+        DevBuf<int32_t> keep_d(n_keep);
+        keep_d.from_host(keep.data(), keep.size());
+        DevBuf<float> out4(4 * size_t(n_keep));
+        check(ud2_final_score(sc, ls.h, &P, keep_d.p, n_keep, out4.p, nullptr), "ud2_final_score");
+        check(hipDeviceSynchronize() == hipSuccess ? UD2_OK : UD2_ERR_HIP, "hipDeviceSynchronize");
+        out4.to_host(sc4.data(), sc4.size());
+        if (p.energy_decomp) {
+            const size_t nd = size_t(n_keep) * max_natom * 5;
+            DevBuf<float> dec(nd);
+            dec.zero(nd);
+            check(ud2_score_decomp(sc, ls.h, &P, keep_d.p, n_keep, max_natom, dec.p, nullptr), "ud2_score_decomp");
+            check(hipDeviceSynchronize() == hipSuccess ? UD2_OK : UD2_ERR_HIP, "hipDeviceSynchronize");
+            dc.resize(nd);
+            dec.to_host(dc.data(), nd);
+        }
+    }
+    std::vector<float> x_h(xf), d_h(df);
+    x.to_host(x_h.data(), x_h.size());
+    dh.to_host(d_h.data(), d_h.size());
+
+    // final energies and affinity (run_score, DockTask.cpp:224-282) and the output JSON
+    std::vector<std::string> poses(n_lig);
+    std::vector<int> npos(n_lig, 0);
+    std::vector<float> intra_r1(n_lig, 0.0f);
+    for (int k = 0; k < n_keep; k++) {
+        const int l = owner[k];
+        const Lig& L = *ligs[l];
+        const int ch = keep[k] - l * nchain;
+        const float intra = sc4[4 * k], inter = sc4[4 * k + 1], pen = sc4[4 * k + 2], bias_e = sc4[4 * k + 3];
+        if (npos[l] == 0) intra_r1[l] = intra;
+        float total = intra + inter;
+        total = total + pen;
+        const float e_inter = total - intra_r1[l];
+        float denom = kWConfIndep * float(L.ntors);
+        denom = 1.0f + denom;
+        const float aff = e_inter / denom;
+        const float en[7] = {aff, total, intra, inter, pen, aff - e_inter, bias_e};
+        std::string& s = poses[l];
+        if (npos[l]++) s += ',';
+        s += "{\"energy\":[";
+        for (int i = 0; i < 7; i++) {
+            if (i) s += ',';
+            put_float(s, en[i]);
+        }
+        s += "],\"coords\":[";
+        const size_t xo = (size_t(atom_off[l]) * nchain + size_t(ch) * L.natom) * 3;
+        for (int i = 0; i < 3 * L.natom; i++) {
+            if (i) s += ',';
+            put_float(s, x_h[xo + i]);
+        }
+        s += "],\"dihedrals\":[";
+        const size_t dof = size_t(tors_off[l]) * nchain + size_t(ch) * L.ntors;
+        for (int t = 0; t < L.ntors; t++) {
+            if (t) s += ',';
+            put_float(s, d_h[dof + t] * kRad2Deg);
+        }
+        s += ']';
+        if (!dc.empty()) {
+            s += ",\"decomp\":[";
+            for (int a = 0; a < L.natom; a++) {
+                if (a) s += ',';
+                s += '[';
+                for (int q = 0; q < 5; q++) {
+                    if (q) s += ',';
+                    put_float(s, dc[(size_t(k) * max_natom + a) * 5 + q]);
+                }
+                s += ']';
+            }
+            s += ']';
+        }
+        s += '}';
+    }
+    std::string out = "{";
+    for (int l = 0; l < n_lig; l++) {
+        if (l) out += ',';
+        put_string(out, ligs[l]->name);
+        out += ":[" + poses[l] + "]";
+    }
+    out += '}';
+    return out;
+}
+
+int run(const Params& p0, const char* json, size_t len, const std::string& out_dir, const std::string& name_json,
+        int32_t* n_files) {
+    Params p = p0;
+    if (p.bias != "no" && p.bias != "pos" && p.bias != "align")  // apply_bias, core.cpp:86-97
+        raise("Not supported bias: " + p.bias + " doesn't belong to (no, pos, align)");
+    apply_search_mode(p);
+    if (p.opt_steps < 0) p.opt_steps = -1;
+    if (p.constraint) p.randomize = false;  // prepare_context_by_input, core.cpp:142-146
+    apply_task(p);
+    JVal doc = Parser(json, len).document();
+    if (doc.kind != JVal::Obj) raise("the input JSON is not an object");
+    if (!out_dir.empty()) {
+        std::error_code ec;
+        std::filesystem::create_directories(out_dir, ec);
+        if (ec) raise("cannot create " + out_dir + ": " + ec.message(), UD2_ERR_STATE);
+    }
+    Rec R = receptor_from_json(doc.find("receptor"), p.lo, p.hi);
+    std::vector<Lig> ligs;
+    for (auto& kv : doc.obj) {
+        if (kv.first == "score" || kv.first == "receptor") continue;
+        ligs.push_back(ligand_from_json(kv.first, kv.second, p.use_tor_lib));
+    }
+    if (ligs.empty()) return UD2_OK;  // "No flexible molecules are found" (json.cpp:71-73)
+    check(ud2_set_device(p.gpu), "ud2_set_device");
+    SceneH sc;
+    check(ud2_scene_create(R.xyz.empty() ? nullptr : R.xyz.data(), R.types.empty() ? nullptr : R.types.data(),
+                           int32_t(R.types.size()), p.lo, p.hi, &sc.h),
+          "ud2_scene_create");
+    if (p.inter_mode == UD2_MODE_GRID) {
+        std::set<int32_t> probes(std::begin(kDefaultProbes), std::end(kDefaultProbes));
+        for (auto& L : ligs)
+            for (int32_t t : L.types)
+                if (t != 0) probes.insert(t);
+        std::vector<int32_t> pv(probes.begin(), probes.end());
+        check(ud2_scene_build_grids(sc.h, kGridSpacing, pv.data(), int32_t(pv.size())), "ud2_scene_build_grids");
+    }
+    // size groups (group_flex_mols, screening.cpp:17-41) and batches of at most 1,000
+    std::vector<std::vector<int>> groups(5);
+    for (int i = 0; i < int(ligs.size()); i++) {
+        int g = 0;
+        while (g < 4 && !(ligs[i].natom <= kGroups[g][0] && ligs[i].ntors <= kGroups[g][1])) g++;
+        groups[g].push_back(i);
+    }
+    int batch_id = 0;
+    int opt_steps = p.opt_steps;
+    for (auto& grp : groups) {
+        for (size_t start = 0; start < grp.size(); start += kMaxBatch) {
+            std::vector<const Lig*> batch;
+            int natom_max = 0;
+            for (size_t k = start; k < std::min(grp.size(), start + kMaxBatch); k++) {
+                batch.push_back(&ligs[grp[k]]);
+                natom_max = std::max(natom_max, ligs[grp[k]].natom);
+            }
+            batch_id++;
+            if (opt_steps == -1) opt_steps = (25 + natom_max) / 3;  // persists (screening.cpp:206-208)
+            std::string text = dock_batch(sc.h, batch, p, opt_steps);
+            std::string name = name_json + "_" + std::to_string(batch_id) + ".json";
+            write_with_retry(out_dir.empty() ? name : out_dir + "/" + name, text);  // gen_filepath, myio.cpp:7-13
+            if (n_files) *n_files = batch_id;
+        }
+    }
+    return UD2_OK;
+}
+
+}  // namespace
+
+struct ud2_handle {
+    Params p;
+};
+
+extern "C" {
+
+int ud2_create(const ud2_run_params* rp, ud2_handle** out) {
+    if (!rp || !out) return ud2::set_error(UD2_ERR_ARG, "ud2_create: null argument");
+    *out = nullptr;
+    if (rp->inter_mode != UD2_MODE_DIRECT && rp->inter_mode != UD2_MODE_GRID)
+        return ud2::set_error(UD2_ERR_ARG, "ud2_create: bad inter_mode");
+    auto* h = new (std::nothrow) ud2_handle;
+    if (!h) return ud2::set_error(UD2_ERR_ARG, "ud2_create: out of memory");
+    Params& p = h->p;
+    for (int d = 0; d < 3; d++) {
+        p.lo[d] = rp->center[d] - rp->size[d] / 2.0f;
+        p.hi[d] = rp->center[d] + rp->size[d] / 2.0f;
+    }
+    p.task = rp->task ? rp->task : "screen";
+    p.search_mode = rp->search_mode ? rp->search_mode : "balance";
+    p.bias = rp->bias ? rp->bias : "no";
+    p.exhaustiveness = rp->exhaustiveness;
+    p.randomize = rp->randomize != 0;
+    p.mc_steps = rp->mc_steps;
+    p.opt_steps = rp->opt_steps;
+    p.refine_steps = rp->refine_steps;
+    p.num_pose = rp->num_pose;
+    p.rmsd_limit = rp->rmsd_limit;
+    p.energy_range = rp->energy_range;
+    p.seed = rp->seed;
+    p.bias_k = rp->bias_k;
+    p.constraint = rp->constraint_docking != 0;
+    p.use_tor_lib = rp->use_tor_lib != 0;
+    p.energy_decomp = rp->energy_decomp != 0;
+    p.gpu = rp->gpu_device_id;
+    p.inter_mode = rp->inter_mode;
+    *out = h;
+    return UD2_OK;
+}
+
+int ud2_run_json(ud2_handle* h, const char* json, size_t len, const char* out_dir, const char* name_json,
+                 int32_t* n_files_h) {
+    if (n_files_h) *n_files_h = 0;
+    if (!h || (!json && len)) return ud2::set_error(UD2_ERR_ARG, "ud2_run_json: null argument");
+    try {
+        return run(h->p, json ? json : "", len, out_dir ? out_dir : "", name_json ? name_json : "from_python_obj",
+                   n_files_h);
+    } catch (const Error& e) {
+        return ud2::set_error(e.code, e.msg);
+    } catch (const std::exception& e) {
+        return ud2::set_error(UD2_ERR_STATE, std::string("ud2_run_json: ") + e.what());
+    }
+}
+
+int ud2_destroy(ud2_handle* h) {
+    delete h;
+    return UD2_OK;
+}
+
+}  // extern "C"

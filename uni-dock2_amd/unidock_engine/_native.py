@@ -57,6 +57,17 @@ class LmcParams(C.Structure):
                 ("sel_temp", C.c_float), ("n_start", C.c_int32), ("collect_x_d", C.c_void_p), ("collect_y_d", C.c_void_p), ("collect_e0_d", C.c_void_p)]
 
 
+class RunParams(C.Structure):
+    """ud2_run_params: the run options of ud2_create (CoreInput / DockingPipeline's arguments)."""
+    _fields_ = [("center", C.c_float * 3), ("size", C.c_float * 3), ("task", C.c_char_p),
+                ("search_mode", C.c_char_p), ("exhaustiveness", C.c_int32), ("randomize", C.c_int32),
+                ("mc_steps", C.c_int32), ("opt_steps", C.c_int32), ("refine_steps", C.c_int32),
+                ("num_pose", C.c_int32), ("rmsd_limit", C.c_float), ("energy_range", C.c_float),
+                ("seed", C.c_int64), ("bias", C.c_char_p), ("bias_k", C.c_float), ("constraint_docking", C.c_int32),
+                ("use_tor_lib", C.c_int32), ("energy_decomp", C.c_int32), ("gpu_device_id", C.c_int32),
+                ("max_gpu_mem", C.c_int32), ("inter_mode", C.c_int32)]
+
+
 _lib = None
 
 
@@ -108,6 +119,9 @@ def lib():
         "ud2_mlp_train_step": (C.c_int, [vp, vp, vp, vp, i32, vp, vp, vp]),
         "ud2_features": (C.c_int, [vp, vp, vp, vp, i32, vp, vp]),
         "ud2_lmc": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "ud2_create": (C.c_int, [vp, vp]),
+        "ud2_run_json": (C.c_int, [vp, C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, vp]),
+        "ud2_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -125,6 +139,7 @@ EXPORTED = (
     "ud2_score_posevecs", "ud2_energy_grad", "ud2_pose_update", "ud2_line_search", "ud2_optimize", "ud2_mc",
     "ud2_cluster", "ud2_final_score", "ud2_score_decomp", "ud2_mlp_create", "ud2_mlp_set_params", "ud2_mlp_get_params",
     "ud2_mlp_destroy", "ud2_mlp_forward", "ud2_mlp_backward", "ud2_mlp_train_step", "ud2_features", "ud2_lmc",
+    "ud2_create", "ud2_run_json", "ud2_destroy",
 )
 
 
@@ -363,6 +378,34 @@ class PoseBuffers:
         if L.ntors:
             self.dihe[do:do + L.ntors] = torch.as_tensor(np.asarray(dihe, np.float32), device=dev)
         self.e[pid] = float(e)
+
+
+def run_json(p: dict, text: str, inter_mode: int = MODE_GRID) -> int:
+    """One whole docking run in the library (ud2_create + ud2_run_json): ``p`` holds
+    DockingPipeline's keyword arguments (box centre and size included), ``text`` the engine-JSON
+    document. Writes <output_dir>/<name_json>_<batch>.json; returns the number of batches."""
+    rp = RunParams()
+    rp.center[:] = [float(p["center_x"]), float(p["center_y"]), float(p["center_z"])]
+    rp.size[:] = [float(p["size_x"]), float(p["size_y"]), float(p["size_z"])]
+    rp.task, rp.search_mode, rp.bias = (str(p[k]).encode() for k in ("task", "search_mode", "bias"))
+    for k in ("exhaustiveness", "mc_steps", "opt_steps", "refine_steps", "num_pose", "gpu_device_id", "max_gpu_mem"):
+        setattr(rp, k, int(p[k]))
+    for k in ("randomize", "constraint_docking", "use_tor_lib", "energy_decomp"):
+        setattr(rp, k, 1 if p[k] else 0)
+    rp.rmsd_limit, rp.energy_range, rp.bias_k = float(p["rmsd_limit"]), float(p["energy_range"]), float(p["bias_k"])
+    rp.seed = int(p["seed"])
+    rp.inter_mode = int(inter_mode)
+    L = lib()
+    h = C.c_void_p()
+    check(L.ud2_create(C.byref(rp), C.byref(h)), "ud2_create")
+    try:
+        raw = text.encode("utf-8")
+        n = C.c_int32(0)
+        check(L.ud2_run_json(h, raw, len(raw), str(p["output_dir"]).encode(), str(p["name_json"]).encode(),
+                             C.byref(n)), "ud2_run_json")
+        return n.value
+    finally:
+        L.ud2_destroy(h)
 
 
 def version() -> str:

@@ -118,7 +118,7 @@ def dumps_poses(results) -> str:
                 extra = ',"decomp":[' + ",".join(
                     "[" + ",".join(fmt_float(x) for x in row) + "]" for row in np.asarray(r.decomp[k], np.float32)) + "]"
             poses.append('{"energy":[' + e + '],"coords":[' + c + '],"dihedrals":[' + d + "]" + extra + "}")
-        parts.append(json.dumps(r.name) + ":[" + ",".join(poses) + "]")
+        parts.append(json.dumps(r.name, ensure_ascii=False) + ":[" + ",".join(poses) + "]")
     return "{" + ",".join(parts) + "}"
 
 
@@ -158,6 +158,8 @@ class DockingPipeline:
                  gpu_device_id=DEFAULTS["gpu_device_id"], name_json=DEFAULTS["name_json"],
                  max_gpu_mem=DEFAULTS["max_gpu_mem"]):
         self.box = Box.from_center_size((center_x, center_y, center_z), (size_x, size_y, size_z))
+        self.center_size = dict(center_x=float(center_x), center_y=float(center_y), center_z=float(center_z),
+                                size_x=float(size_x), size_y=float(size_y), size_z=float(size_z))
         self.params = dict(output_dir=str(output_dir), task=task, search_mode=search_mode,
                            exhaustiveness=int(exhaustiveness), randomize=bool(randomize), mc_steps=int(mc_steps),
                            opt_steps=int(opt_steps), refine_steps=int(refine_steps), num_pose=int(num_pose),
@@ -168,6 +170,9 @@ class DockingPipeline:
         self.json_data: dict = {}
         self.output_files: list = []
         self.inter_mode = N.MODE_GRID
+        # "native": the whole run in libud2.so (ud2_run_json); "python": the same pipeline driven
+        # from Python over the kernel-level entry points (core_pipeline below)
+        self.host = "native"
 
     def set_receptor(self, receptor_info: list) -> None:
         self.json_data["receptor"] = receptor_info
@@ -185,17 +190,40 @@ class DockingPipeline:
 
     # ------------------------------------------------------------------
     def _run(self) -> None:
-        # the reference serialises the cached objects to JSON text and parses them back
-        doc = json.loads(json.dumps(self.json_data))
-        self.output_files.extend(core_pipeline(dict(self.params), self.box, doc, self.inter_mode))
+        # the reference serialises the cached objects to JSON text and hands the text to its C++
+        # core_pipeline (pipeline.cpp:120-145)
+        text = json.dumps(self.json_data, ensure_ascii=False)
+        if self.host == "python":
+            self.output_files.extend(core_pipeline(dict(self.params), self.box, json.loads(text), self.inter_mode))
+            return
+        if self.host != "native":
+            raise ValueError(f"unknown host {self.host!r} (native, python)")
+        self.output_files.extend(run_native(dict(self.params, **self.center_size), text, self.inter_mode))
+
+
+def output_path(p: dict, batch_id: int) -> str:
+    """<output_dir>/<name_json>_<batch_id>.json; an empty directory is the working directory
+    (gen_filepath, src/myutils/myio.cpp:7-13)."""
+    name = f"{p['name_json']}_{batch_id}.json"
+    return os.path.join(p["output_dir"], name) if p["output_dir"] else name
+
+
+def run_native(p: dict, text: str, inter_mode: int = N.MODE_GRID) -> list:
+    """core_pipeline in libud2.so (ud2_run_json) over an engine-JSON text; ``p`` holds
+    DockingPipeline's keyword arguments with the box centre and size. Returns the files written."""
+    t0 = time.time()
+    n = N.run_json(p, text, inter_mode)
+    log.info("%d batch(es) docked in %.1f ms", n, (time.time() - t0) * 1e3)
+    return [output_path(p, k) for k in range(1, n + 1)]
 
 
 def core_pipeline(p: dict, box: Box, doc: dict, inter_mode: int = N.MODE_GRID) -> list:
     """core_pipeline, src/screening/core.cpp:100-223, over an engine-JSON document (receptor under
     "receptor", one member per ligand, optional "score"): bias check, search-mode and task presets,
     size groups and batches of at most 1,000 ligands, one docking run and one output file per batch.
-    ``p`` holds DockingPipeline's keyword arguments. Returns the files written; raises on failure
-    (both callers turn that into the reference's failure convention)."""
+    ``p`` holds DockingPipeline's keyword arguments. Returns the files written; raises on failure.
+    This is the Python-driven twin of ud2_run_json (DockingPipeline.host = "python"); both make
+    the same kernel calls with the same arguments and write the same bytes."""
     p = dict(p)
     if p["bias"] not in ("no", "pos", "align"):  # apply_bias, core.cpp:86-97
         raise ValueError(f"Not supported bias: {p['bias']} doesn't belong to (no, pos, align)")
@@ -233,9 +261,7 @@ def core_pipeline(p: dict, box: Box, doc: dict, inter_mode: int = N.MODE_GRID) -
             t0 = time.time()
             ls = N.LigSet(batch, bias=p["bias"], bias_k=p["bias_k"], constraint=p["constraint_docking"])
             results, _ = eng.dock(ls, prm)
-            # gen_filepath, src/myutils/myio.cpp:7-13: an empty directory means the working directory
-            name = f"{p['name_json']}_{batch_id}.json"
-            path = os.path.join(p["output_dir"], name) if p["output_dir"] else name
+            path = output_path(p, batch_id)
             write_json_with_retry(path, dumps_poses(results))
             files.append(path)
             log.info("Batch %d (%d ligands) running time: %.1f ms", batch_id, len(batch),

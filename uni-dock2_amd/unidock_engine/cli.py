@@ -19,14 +19,12 @@ As in main.cpp:89-99, the argument right after the config path is skipped.
 """
 from __future__ import annotations
 
-import json
 import logging
 import os
 import sys
 
 import yaml
 
-from .topology import Box
 
 log = logging.getLogger("unidock_engine")
 
@@ -152,8 +150,9 @@ def _mapping(node) -> dict:
 
 
 def read_config(path: str):
-    """Parse a ``ud2`` YAML config (main.cpp:118-165). Returns (params, box, json_path): params holds
-    DockingPipeline's keyword arguments except the box and name_json. Raises ConfigError."""
+    """Parse a ``ud2`` YAML config (main.cpp:118-165). Returns (params, json_path): params holds
+    DockingPipeline's keyword arguments (box centre and size included) except name_json.
+    Raises ConfigError."""
     with open(path) as f:
         root = yaml.compose(f, Loader=yaml.SafeLoader)
     if root is not None and not isinstance(root, yaml.MappingNode):
@@ -168,10 +167,8 @@ def read_config(path: str):
         else:
             v = _convert(node, typ, f"{sec}.{key}")
         vals[FIELD.get(key, key)] = v
-    box = Box.from_center_size((vals.pop("center_x"), vals.pop("center_y"), vals.pop("center_z")),
-                               (vals.pop("size_x"), vals.pop("size_y"), vals.pop("size_z")))
     fp_json = vals.pop("json")
-    return vals, box, fp_json
+    return vals, fp_json
 
 
 def _yaml_scalar(v) -> str:
@@ -293,7 +290,7 @@ def _run_config(fp_config: str) -> int:
     """main.cpp:116-192: read the config and the input JSON, run the pipeline."""
     log.info("Using config file: %s", fp_config)
     try:
-        params, box, fp_json = read_config(fp_config)
+        params, fp_json = read_config(fp_config)
     except (ConfigError, OSError, yaml.YAMLError) as e:
         log.critical("%s", e)
         return 1
@@ -306,15 +303,15 @@ def _run_config(fp_config: str) -> int:
         return 1
     params["name_json"] = name[:-5]
     try:
-        with open(fp_json) as f:
-            doc = json.load(f)
+        with open(fp_json, encoding="utf-8") as f:
+            text = f.read()
     except (OSError, ValueError) as e:  # read_ud_from_json, src/format/json.cpp:24-41
-        log.critical("Failed to parse JSON file %s: %s", fp_json, e)
+        log.critical("Failed to read JSON file %s: %s", fp_json, e)
         return 1
 
-    from .api.python.pipeline import core_pipeline
+    from .api.python.pipeline import run_native
     try:
-        core_pipeline(params, box, doc)
+        run_native(params, text)
     except Exception as e:  # core_pipeline returns 1 (src/screening/core.cpp:218-222)
         log.critical("Core pipeline failed: %s", e)
         log.critical("Docking pipeline failed")
