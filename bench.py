@@ -58,11 +58,12 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=2, help="ligands timed on the CPU oracle (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["batch", "c4", "score", "copy"], default="batch",
+    ap.add_argument("--workload", choices=["batch", "c4", "score", "copy", "gather"], default="batch",
                     help="batch: docking throughput (headline); c4: the large uneven batch (4,096 ligands, 10-80 "
                          "heavy atoms, 0-20 rotatable bonds, 256 chains x 40 steps for both engines, one timed pass "
                          "each after a 16-ligand warmup); score: E+gradient microbenchmark; copy: a 256 MiB device "
-                         "copy that calibrates the FETCH_SIZE / WRITE_SIZE counters")
+                         "copy that calibrates the FETCH_SIZE / WRITE_SIZE counters; gather: random 32-byte map "
+                         "gathers (the gather ceiling)")
     ap.add_argument("--search", choices=["learned", "plain"], default="learned",
                     help="learned: headline (learned vs plain); plain: the plain engine only")
     ap.add_argument("--learned-nchain", type=int, default=36)
@@ -77,6 +78,19 @@ def parse():
                     help="learned: skip the plain engine at the learned engine's chain / step budget")
     ap.add_argument("--n-pose", type=int, default=1 << 20)
     return ap.parse_args()
+
+
+def host_cpu() -> dict:
+    """The host's CPU model and the cores this process may use (lscpu / nproc)."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
 
 
 def cpu_baseline(ligands, receptor, box, nchain, mc_steps, opt_steps, n_sample, threads):
@@ -102,12 +116,14 @@ def cpu_baseline(ligands, receptor, box, nchain, mc_steps, opt_steps, n_sample, 
         "sample": (f"first {n_sample} ligands of the batch, full pipeline ({nchain} chains x {mc_steps} MC steps, "
                    f"BFGS {opt_steps}, cluster, refine, score), direct receptor sum, FP32, {threads} threads, "
                    f"{secs:.1f} s, {evals} energy evaluations"),
+        "host": host_cpu(),
     }
 
 
 def score_microbench(args):
     """E + gradient of 1,048,576 random poses of the 40-atom, 8-torsion ligand (grid inter term,
-    curl cap 10): evals/s and the gather roofline (32 B of map corners per heavy atom per eval)."""
+    curl cap 10): evals/s and the HBM roofline of 1,400 algorithmic bytes per eval (32 B of map
+    corners per heavy atom, 60 B of pose vector in, 60 B of energy and gradient out)."""
     import numpy as np
     import torch
     from unidock_engine import _native as N
@@ -147,7 +163,9 @@ def score_microbench(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     ms = np.asarray([a.elapsed_time(b) for a, b in evs])
-    bytes_per_eval = 32 * lig.n_heavy
+    # algorithmic bytes per eval: 8 map corners per heavy atom, the pose vector in (centre, quaternion,
+    # torsions) and the energy and gradient out
+    bytes_per_eval = 32 * lig.n_heavy + 4 * (7 + lig.ntors) + 4 * (1 + 6 + lig.ntors)
     achieved = n * bytes_per_eval / (ms.mean() / 1000.0) / 1e9
     out = {
         "metric": "energy+gradient evals/sec (scoring microbenchmark)",
@@ -170,6 +188,11 @@ def score_microbench(args):
                      "traffic": None, "algorithmic_bytes_per_launch": int(n * bytes_per_eval)},
         "cpu_baseline": None,
     }
+    if n == 1 << 20 and args.mode == "grid":
+        t = profiled_traffic("posevec_traffic_bytes_per_launch", SCORE_TRAFFIC_SUMMARY, "1048576 poses score grid")
+        if t:
+            out["roofline"]["traffic"] = t
+            out["roofline"]["traffic_source"] = "profiles/posevec_kernel_c2_score.md (FETCH_SIZE + WRITE_SIZE)"
     if not args.no_cpu:
         sys.path.insert(0, ROOT)
         from oracle import oracle as O
@@ -179,8 +202,62 @@ def score_microbench(args):
         _, _, dt = O.score_posevecs(O.Topo(lig), osc, pv[:m].cpu().numpy(), v_cap=10.0, slope=1e6,
                                     nthreads=threads, sfx="f32")
         out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "evals/s", "cores": threads, "kind": "port",
-                               "sample": f"{m} of the poses, direct receptor sum, FP32, {threads} threads, {dt:.1f} s"}
+                               "sample": f"{m} of the poses, direct receptor sum, FP32, {threads} threads, {dt:.1f} s",
+                               "host": host_cpu()}
     print(json.dumps(out), flush=True)
+
+
+def gather_probe(scene, span_cells: int = 0, n_gathers: int = 1 << 27, reps: int = 5) -> dict:
+    """Random 32-byte gathers over the scene's cell-major maps (ud2_scene_gather_probe): the
+    bandwidth scattered map reads can reach over that footprint, timed with HIP events."""
+    import numpy as np
+    import torch
+    from unidock_engine import _native as N
+    n_thr = (n_gathers + 15) // 16
+    out = torch.empty(n_thr, dtype=torch.float32, device="cuda")
+    st = N.stream_ptr()
+
+    def run(seed):
+        N.check(N.lib().ud2_scene_gather_probe(scene.h, n_gathers, span_cells, seed, C.c_void_p(out.data_ptr()), st),
+                "ud2_scene_gather_probe")
+
+    run(0)
+    ms = []
+    for r in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        run(r + 1)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    gn, _, _, n_types = scene.grid_info()
+    total = int(gn[0] - 1) * int(gn[1] - 1) * int(gn[2] - 1) * int(n_types)
+    span = span_cells if span_cells > 0 else total
+    return {"achieved": round(n_gathers * 32 / (float(np.median(ms)) / 1e3) / 1e9, 1), "unit": "GB/s",
+            "footprint_bytes": int(span * 32), "gathers": n_gathers, "ms": round(float(np.median(ms)), 3)}
+
+
+def gather_microbench(args):
+    """Random-gather ceiling of the grid maps at three footprints: all maps (the search's), one map,
+    and 4 MB (one XCD's L2)."""
+    from unidock_engine import _native as N
+    from unidock_engine import synthetic as S
+    N.check(N.lib().ud2_set_device(0), "ud2_set_device")
+    scene = N.Scene(S.make_receptor(), S.docking_box())
+    scene.build_grids(S.GRID_SPACING, [2, 3, 4, 5, 6, 10, 11, 15])
+    gn, _, _, n_types = scene.grid_info()
+    per_map = int(gn[0] - 1) * int(gn[1] - 1) * int(gn[2] - 1)
+    rows = {name: gather_probe(scene, span) for name, span in
+            (("all_maps", 0), ("one_map", per_map), ("l2_4MB", (4 << 20) // 32))}
+    full = rows["all_maps"]
+    print(json.dumps({
+        "metric": "random 32-byte map gathers, GB/s", "value": full["achieved"], "unit": "GB/s", "n_gpus": 1,
+        "steps": 5, "warmup": 1, "ms_per_step": full["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic receptor grid maps",
+        "config": {"workload": f"{full['gathers']} random cell gathers (two float4 each) per launch", **rows},
+        "roofline": {"bound": "hbm", "achieved": full["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(full["achieved"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None}), flush=True)
 
 
 def copy_probe():
@@ -196,35 +273,37 @@ def copy_probe():
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "lmc_mc_kernels_c1_batch_traffic.md")
+SCORE_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "posevec_kernel_c2_score.md")
 
 
 DEFAULT_TRAFFIC_CONFIG = "1024 ligands grid; learned 36x40 starts 4 cand 16 T 1.0; plain 128x40"
 
 
 def kernel_sources_digest() -> str:
-    """SHA-1 (12 hex digits) of the kernel sources: a traffic summary counts only for the build it measured."""
+    """SHA-1 (12 hex digits) of the sources that define and launch the kernels (the device code and
+    ud2_api.cpp; not the whole-run orchestration): a traffic summary counts only for the build it measured."""
     import hashlib
     d = os.path.join(ROOT, "uni-dock2_amd", "csrc")
     h = hashlib.sha1()
     for f in sorted(os.listdir(d)):
-        if f.endswith((".h", ".hip", ".cpp")):
+        if f.endswith((".h", ".hip")) or f == "ud2_api.cpp":
             h.update(f.encode())
             h.update(open(os.path.join(d, f), "rb").read())
     return h.hexdigest()[:12]
 
 
-def profiled_traffic(key: str):
-    """Bytes per launch measured with the FETCH_SIZE / WRITE_SIZE counters for the default
-    workload (profiles/), or None when that summary is for another configuration or build."""
+def profiled_traffic(key: str, summary: str = TRAFFIC_SUMMARY, config: str = DEFAULT_TRAFFIC_CONFIG):
+    """Bytes per launch measured with the FETCH_SIZE / WRITE_SIZE counters for a workload
+    (profiles/), or None when that summary is for another configuration or build."""
     vals = {}
     try:
-        for line in open(TRAFFIC_SUMMARY):
+        for line in open(summary):
             if "=" in line and not line.startswith(("|", "`", " ")):
                 k, v = line.strip().split("=", 1)
                 vals[k] = v
     except OSError:
         return None
-    if vals.get("config") != DEFAULT_TRAFFIC_CONFIG or vals.get("sources") != kernel_sources_digest() or key not in vals:
+    if vals.get("config") != config or vals.get("sources") != kernel_sources_digest() or key not in vals:
         return None
     return int(float(vals[key]))
 
@@ -257,6 +336,8 @@ def main():
         return score_microbench(args)
     if args.workload == "copy":
         return copy_probe()
+    if args.workload == "gather":
+        return gather_microbench(args)
     import numpy as np
     from unidock_engine import _native as N
     from unidock_engine import proposal as LP
@@ -390,6 +471,11 @@ def main():
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
     }
+    if args.mode == "grid":
+        # the measured ceiling of random 32-byte gathers over the same maps (outside the timed region)
+        gc = gather_probe(eng.scene)
+        out["roofline"]["gather_ceiling"] = gc
+        out["roofline"]["frac_of_gather_ceiling"] = round(achieved / gc["achieved"], 6)
     default_run = (args.n_lig, args.mode, args.nchain, args.mc_steps, args.learned_nchain, args.learned_steps,
                    args.n_cand, args.sel_temp, args.n_start) == (1024, "grid", 128, 40, 36, 40, 16, 1.0, 4)
     if default_run:

@@ -105,6 +105,14 @@ int ud2_scene_set_grids(ud2_scene* sc, const int32_t n_h[3], const float origin_
                         const int32_t* probe_types_h, int32_t n_types, const float* maps_h);
 int ud2_scene_grid_info(const ud2_scene* sc, int32_t n_h[3], float origin_h[3], float* spacing_h, int32_t* n_types_h);
 int ud2_scene_download_maps(const ud2_scene* sc, float* maps_h);
+/* Random-gather probe (no reference counterpart; a measurement aid): ceil(n_gathers / 16)
+ * threads each read 16 cells of the cell-major map copy the search kernels gather from (two
+ * float4, one 32-byte sector per cell), drawn by a hash of (thread, seed) from the first
+ * span_cells cells (<= 0: all of them, 8 maps x 80^3 cells = 131 MB for the default grids),
+ * and write the sum of what they read to out_d[thread]. Its achieved bandwidth is the ceiling
+ * for scattered map gathers over that footprint. */
+int ud2_scene_gather_probe(const ud2_scene* sc, int64_t n_gathers, int64_t span_cells, uint32_t seed, float* out_d,
+                           void* stream);
 int ud2_scene_destroy(ud2_scene* sc);
 
 /* ------------------------------------------------------------- ligands

@@ -438,6 +438,18 @@ int ud2_scene_download_maps(const ud2_scene* sc, float* maps_h) {
     return UD2_OK;
 }
 
+int ud2_scene_gather_probe(const ud2_scene* sc, int64_t n_gathers, int64_t span_cells, uint32_t seed, float* out_d,
+                           void* stream) {
+    if (!sc || !out_d || n_gathers <= 0) return fail(UD2_ERR_ARG, "bad arguments");
+    if (!sc->d_cells) return fail(UD2_ERR_STATE, "the scene has no grid maps");
+    const int64_t total = int64_t(sc->gn[0] - 1) * (sc->gn[1] - 1) * (sc->gn[2] - 1) * int64_t(sc->probe_types.size());
+    const int64_t span = span_cells > 0 ? span_cells : total;
+    if (span > total || span > int64_t(UINT32_MAX)) return fail(UD2_ERR_ARG, "span beyond the maps");
+    const int64_t n_threads = (n_gathers + kProbePerThread - 1) / kProbePerThread;
+    UD2_HIP(launch_gather_probe(sc->d_cells, uint32_t(span), seed, out_d, n_threads, static_cast<hipStream_t>(stream)));
+    return UD2_OK;
+}
+
 int ud2_scene_destroy(ud2_scene* sc) {
     if (!sc) return UD2_OK;
     if (sc->d_rec) hipFree(sc->d_rec);

@@ -240,6 +240,9 @@ struct FeatArgs {
 // Launchers (ud2_kernels.hip); each returns a hipError_t.
 hipError_t launch_grid_build(const GridArgs& a, hipStream_t s);
 hipError_t launch_brick(const BrickArgs& a, hipStream_t s);
+constexpr int kProbePerThread = 16;  // gathers per thread of the gather probe
+hipError_t launch_gather_probe(const float4* cells, uint32_t span, uint32_t seed, float* out, int64_t n_threads,
+                               hipStream_t s);
 hipError_t launch_mc(McArgs a, int apl, int n_blocks, hipStream_t s);
 hipError_t launch_opt(OptArgs a, int apl, hipStream_t s);
 hipError_t launch_posevec(PoseVecArgs a, int apl, int n_blocks, hipStream_t s);

@@ -405,6 +405,36 @@ hipError_t launch_brick(const BrickArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Random-gather probe over the cell-major maps: each thread reads kProbePerThread cells drawn by an
+// integer hash from the first `span` cells (two float4 per cell, one 32-byte sector, as the search's
+// trilinear gathers read them) and writes the sum of what it read. It measures the bandwidth that
+// scattered 32-byte gathers over this footprint can reach on the device.
+__global__ __launch_bounds__(256) void gather_probe_kernel(const float4* cells, uint32_t span, uint32_t seed,
+                                                           float* out, int64_t n_threads) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n_threads) return;
+    float acc = 0.0f;
+    uint32_t h = uint32_t(id) ^ (uint32_t(id >> 32) * 0x85EBCA6Bu) ^ seed;
+#pragma unroll 8
+    for (int k = 0; k < kProbePerThread; k++) {
+        // murmur3's 32-bit finaliser of (h + k)
+        uint32_t x = h + uint32_t(k) * 0x9E3779B9u;
+        x ^= x >> 16; x *= 0x85EBCA6Bu; x ^= x >> 13; x *= 0xC2B2AE35u; x ^= x >> 16;
+        const uint32_t c = uint32_t((uint64_t(x) * span) >> 32);  // < span
+        const float4 a = cells[2 * (size_t)c], b = cells[2 * (size_t)c + 1];
+        acc += (a.x + a.y + a.z + a.w) + (b.x + b.y + b.z + b.w);
+    }
+    out[id] = acc;
+}
+
+hipError_t launch_gather_probe(const float4* cells, uint32_t span, uint32_t seed, float* out, int64_t n_threads,
+                               hipStream_t s) {
+    if (n_threads <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_probe_kernel, dim3((unsigned)((n_threads + 255) / 256)), dim3(256), 0, s, cells, span,
+                       seed, out, n_threads);
+    return hipGetLastError();
+}
+
 hipError_t launch_grid_build(const GridArgs& a, hipStream_t s) {
     int64_t npt = (int64_t)a.S.gn[0] * a.S.gn[1] * a.S.gn[2];
     hipLaunchKernelGGL(grid_kernel, dim3((unsigned)((npt + 255) / 256)), dim3(256), 0, s, a);
