@@ -201,8 +201,9 @@ int ud2_score_decomp(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses*
  * ud2_run_json parses an engine-JSON document (receptor under "receptor", one
  * member per ligand, optional "score"; E/src/format/json.cpp:34-76 and
  * E/src/format/rapidjson_parser.cpp:25-296), applies the search-mode and task
- * presets, docks the ligands in size groups and batches of at most 1,000
- * (E/src/screening/screening.cpp:140-219) and writes
+ * presets, docks the ligands in size groups and batches of at most 1,000 within
+ * the device-memory budget (E/src/screening/screening.cpp:140-219,
+ * E/src/screening/core.cpp:32-63) and writes
  * <out_dir>/<name_json>_<batch>.json (out_dir "" = working directory) in the
  * reference's output format (E/src/format/json.cpp:88-191). *n_files_h (optional)
  * = batches written. Host memory only crosses the ABI; the device work runs on
@@ -228,7 +229,8 @@ typedef struct {
     int32_t use_tor_lib;
     int32_t energy_decomp;
     int32_t gpu_device_id;
-    int32_t max_gpu_mem;        /* MB; accepted, the batches are bounded by count (288 GB per device) */
+    int32_t max_gpu_mem;        /* MB (<= 0: none): with 95 % of the free device memory and of the host's
+                                   memory, the budget that bounds each batch's estimated device memory */
     int32_t inter_mode;         /* UD2_MODE_GRID or UD2_MODE_DIRECT for the search and refinement */
 } ud2_run_params;
 

@@ -47,7 +47,23 @@ CASES = {
     "task_mc": ("synthetic", dict(task="mc", search_mode="free", exhaustiveness=16, mc_steps=8, seed=2), 1),
     "task_randomize": ("5S8I", dict(task="randomize", search_mode="free", exhaustiveness=16, seed=9), 1),
     "task_optimize": ("synthetic", dict(task="optimize", seed=4), 0),
+    # a device-memory budget 2 MB above the scene's share: batches of a few ligands each
+    "memory_budget": ("synthetic", dict(search_mode="free", exhaustiveness=1024, mc_steps=4, refine_steps=1, seed=8,
+                                        max_gpu_mem="scene+2"), 1),
 }
+
+
+def scene_mb(gpu, rec, box_kw):
+    """The scene's device memory in MB as the pipeline counts it (receptor, maps, cell-major copy)."""
+    from unidock_engine import synthetic as S
+    from unidock_engine.api.python import pipeline
+    from unidock_engine.topology import Box, receptor_from_json
+    box = Box.from_center_size((box_kw["center_x"], box_kw["center_y"], box_kw["center_z"]),
+                               (box_kw["size_x"], box_kw["size_y"], box_kw["size_z"]))
+    r = receptor_from_json(rec, box)
+    sc = gpu.Scene(r, box)
+    sc.build_grids(S.GRID_SPACING, [2, 3, 4, 5, 6, 10, 11, 15])
+    return pipeline.scene_bytes(r, sc) // pipeline.MIB
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -62,6 +78,8 @@ def test_native_run_matches_python_driven_run(gpu, tmp_path, case):
         box = BOX_BIAS
     else:
         rec, ligs, box = synthetic_doc()
+    if isinstance(kw.get("max_gpu_mem"), str):
+        kw = dict(kw, max_gpu_mem=scene_mb(gpu, rec, box) + int(kw["max_gpu_mem"].split("+")[1]))
     print(f"host parity case {case}", flush=True)
     outs = {}
     for run, host in (("native", "native"), ("native2", "native"), ("python", "python")):
@@ -83,6 +101,8 @@ def test_native_run_matches_python_driven_run(gpu, tmp_path, case):
         assert a == b, (name, len(a), len(b), i, a[max(0, (i or 0) - 120):(i or 0) + 80])
     doc = json.loads(next(iter(outs["native"].values())))
     assert doc and all(isinstance(v, list) for v in doc.values())
+    if case == "memory_budget":
+        assert len(outs["native"]) >= 3, sorted(outs["native"])
     if kw.get("energy_decomp"):
         assert all("decomp" in pose for poses in doc.values() for pose in poses)
 
@@ -93,7 +113,7 @@ def test_native_run_failures(gpu, tmp_path):
     from unidock_engine.api.python import pipeline
     rec, ligs = golden("5S8I_unidock2.json")
     for kw, data in ((dict(task="dock"), ligs), (dict(search_mode="turbo"), ligs),
-                     (dict(), {"bad": {"atoms": [[0.0, 1.0]]}})):
+                     (dict(), {"bad": {"atoms": [[0.0, 1.0]]}}), (dict(max_gpu_mem=1), ligs)):
         p = pipeline.DockingPipeline(output_dir=str(tmp_path), **BOX_5S8I, **kw)
         p.set_receptor(rec)
         p.add_ligands(data)

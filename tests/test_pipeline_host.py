@@ -133,3 +133,25 @@ def test_native_run_rejects_bad_options_and_input(tmp_path):
     # no ligands: nothing written, no error
     assert N.run_json(_native_params(output_dir=out, name_json="x"), '{"receptor": [], "score": ["vina"]}') == 0
     assert not list(tmp_path.iterdir())
+
+
+def test_memory_budget_batches():
+    """Batches of a size group: at most 1,000 ligands, estimated device memory under the budget
+    (run_screening, screening.cpp:168-196); a budget below one ligand is an error."""
+    from unidock_engine import synthetic as S
+    ligs = [s.ligand for s in S.config1_ligands(12)]
+    group = list(range(len(ligs)))
+    one = [P.BatchSize().add(l).bytes(4096) for l in ligs]
+    assert all(1 << 19 < b < 4 << 20 for b in one)  # 0.5-4 MB per ligand at 4,096 chains
+    batches = list(P.memory_batches(ligs, group, 4096, 8))
+    assert [i for b in batches for i in b] == group and len(batches) > 2
+    for b in batches:
+        est = P.BatchSize()
+        for i in b:
+            est = est.add(ligs[i])
+        assert est.bytes(4096) // P.MIB < 8
+    assert list(P.memory_batches(ligs, group, 4096, 10 ** 6)) == [group]
+    big = list(range(2500))
+    assert [len(b) for b in P.memory_batches([ligs[0]] * 2500, big, 1, 10 ** 6)] == [1000, 1000, 500]
+    with pytest.raises(RuntimeError, match="Not enough GPU Memory"):
+        list(P.memory_batches(ligs, group, 4096, 1))

@@ -485,7 +485,7 @@ struct Params {
     bool randomize, constraint, use_tor_lib, energy_decomp;
     float rmsd_limit, energy_range, bias_k;
     int64_t seed;
-    int gpu, inter_mode;
+    int gpu, inter_mode, max_gpu_mem;
 };
 
 // apply_search_mode, core.cpp:65-84
@@ -842,6 +842,39 @@ std::string dock_batch(ud2_scene* sc, const std::vector<const Lig*>& ligs, const
     return out;
 }
 
+constexpr int64_t kMiB = 1048576;
+
+// Device memory of one batch in this library's layout (the counterpart of predict_gpu_flex,
+// screening.cpp:53-127): pose buffers, topology and the clustering / selection arrays.
+struct BatchSize {
+    int64_t n_lig = 0, natom = 0, ntors = 0, npair = 0, nbias = 0;
+    void add(const Lig& L) {
+        n_lig++;
+        natom += L.natom;
+        ntors += L.ntors;
+        npair += int64_t(L.pairs.size() / 2);
+        nbias += int64_t(L.bias.size() / 5);
+    }
+    int64_t bytes(int64_t nchain) const {
+        const int64_t poses = nchain * (3 * natom + ntors) * 4 + n_lig * nchain * 7 * 4;
+        const int64_t topo = natom * 48 + ntors * 64 + npair * 8 + nbias * 32 + n_lig * 128;
+        const int64_t cluster = n_lig * nchain * 4 * 3;
+        return poses + topo + cluster;
+    }
+};
+
+// decide_memory_limit_mb (core.cpp:32-63): 95 % of the device's free memory, capped by 95 % of
+// the host's memory and by the caller's limit when that is positive.
+int64_t memory_limit_mb(int64_t user_mb) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) raise("hipMemGetInfo failed", UD2_ERR_HIP);
+    int64_t lim = int64_t(double(free_b / kMiB) * 0.95);
+    const int64_t ram = int64_t(double(int64_t(sysconf(_SC_PHYS_PAGES)) * sysconf(_SC_PAGE_SIZE) / kMiB) * 0.95);
+    if (ram < lim) lim = ram;
+    if (user_mb > 0 && user_mb < lim) lim = user_mb;
+    return lim;
+}
+
 int run(const Params& p0, const char* json, size_t len, const std::string& out_dir, const std::string& name_json,
         int32_t* n_files) {
     Params p = p0;
@@ -878,7 +911,23 @@ int run(const Params& p0, const char* json, size_t len, const std::string& out_d
         std::vector<int32_t> pv(probes.begin(), probes.end());
         check(ud2_scene_build_grids(sc.h, kGridSpacing, pv.data(), int32_t(pv.size())), "ud2_scene_build_grids");
     }
-    // size groups (group_flex_mols, screening.cpp:17-41) and batches of at most 1,000
+    // device memory budget (decide_memory_limit_mb, core.cpp:32-63), less the scene's share
+    // (predict_gpu_fix, screening.cpp:44-51): receptor, maps and their cell-major copy
+    int32_t gn[3] = {0, 0, 0}, n_types = 0;
+    float origin[3], spacing = 0.0f;
+    if (p.inter_mode == UD2_MODE_GRID) check(ud2_scene_grid_info(sc.h, gn, origin, &spacing, &n_types), "grid info");
+    const int64_t fix_bytes = int64_t(R.types.size()) * 16 +
+                              int64_t(n_types) * gn[0] * gn[1] * gn[2] * 4 +
+                              (n_types ? int64_t(n_types) * (gn[0] - 1) * (gn[1] - 1) * (gn[2] - 1) * 32 : 0);
+    const int64_t fix_mb = fix_bytes / kMiB;
+    const int64_t limit_mb = memory_limit_mb(p.max_gpu_mem);
+    if (limit_mb < fix_mb)
+        raise("GPU memory (" + std::to_string(limit_mb) + " MB) is not enough for the receptor (" +
+                  std::to_string(fix_mb) + " MB)",
+              UD2_ERR_LIMIT);
+    const int64_t batch_limit_mb = limit_mb - fix_mb;
+    // size groups (group_flex_mols, screening.cpp:17-41), then batches of at most 1,000 ligands whose
+    // estimated device memory stays under the budget (screening.cpp:168-196)
     std::vector<std::vector<int>> groups(5);
     for (int i = 0; i < int(ligs.size()); i++) {
         int g = 0;
@@ -888,14 +937,25 @@ int run(const Params& p0, const char* json, size_t len, const std::string& out_d
     int batch_id = 0;
     int opt_steps = p.opt_steps;
     for (auto& grp : groups) {
-        for (size_t start = 0; start < grp.size(); start += kMaxBatch) {
+        size_t done = 0;
+        while (done < grp.size()) {
             std::vector<const Lig*> batch;
+            BatchSize est;
             int natom_max = 0;
-            for (size_t k = start; k < std::min(grp.size(), start + kMaxBatch); k++) {
-                batch.push_back(&ligs[grp[k]]);
-                natom_max = std::max(natom_max, ligs[grp[k]].natom);
-            }
             batch_id++;
+            while (done + batch.size() < grp.size()) {
+                const Lig& L = ligs[grp[done + batch.size()]];
+                BatchSize next = est;
+                next.add(L);
+                if (batch.size() >= size_t(kMaxBatch) || next.bytes(p.exhaustiveness) / kMiB >= batch_limit_mb) break;
+                est = next;
+                batch.push_back(&L);
+                natom_max = std::max(natom_max, L.natom);
+            }
+            if (batch.empty())
+                raise("Not enough GPU Memory to process at least one ligand in batch " + std::to_string(batch_id),
+                      UD2_ERR_LIMIT);
+            done += batch.size();
             if (opt_steps == -1) opt_steps = (25 + natom_max) / 3;  // persists (screening.cpp:206-208)
             std::string text = dock_batch(sc.h, batch, p, opt_steps);
             std::string name = name_json + "_" + std::to_string(batch_id) + ".json";
@@ -944,6 +1004,7 @@ int ud2_create(const ud2_run_params* rp, ud2_handle** out) {
     p.energy_decomp = rp->energy_decomp != 0;
     p.gpu = rp->gpu_device_id;
     p.inter_mode = rp->inter_mode;
+    p.max_gpu_mem = rp->max_gpu_mem;
     *out = h;
     return UD2_OK;
 }

@@ -86,6 +86,70 @@ def group_ligands(ligands) -> list:
     return groups
 
 
+MIB = 1048576
+
+
+class BatchSize:
+    """Device memory of one batch in this library's layout (the counterpart of predict_gpu_flex,
+    src/screening/screening.cpp:53-127): pose buffers, topology, clustering / selection arrays."""
+
+    def __init__(self):
+        self.n_lig = self.natom = self.ntors = self.npair = self.nbias = 0
+
+    def add(self, lig) -> "BatchSize":
+        b = BatchSize()
+        b.n_lig, b.natom, b.ntors = self.n_lig + 1, self.natom + lig.natom, self.ntors + lig.ntors
+        b.npair = self.npair + lig.npair
+        b.nbias = self.nbias + (0 if lig.bias is None else len(lig.bias))
+        return b
+
+    def bytes(self, nchain: int) -> int:
+        poses = nchain * (3 * self.natom + self.ntors) * 4 + self.n_lig * nchain * 7 * 4
+        topo = self.natom * 48 + self.ntors * 64 + self.npair * 8 + self.nbias * 32 + self.n_lig * 128
+        return poses + topo + self.n_lig * nchain * 4 * 3
+
+
+def memory_limit_mb(user_mb: int) -> int:
+    """decide_memory_limit_mb (src/screening/core.cpp:32-63): 95 % of the device's free memory,
+    capped by 95 % of the host's memory and by the caller's limit when that is positive."""
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    lim = int((free // MIB) * 0.95)
+    ram = int((os.sysconf("SC_PHYS_PAGES") * os.sysconf("SC_PAGE_SIZE") // MIB) * 0.95)
+    lim = min(lim, ram)
+    if user_mb > 0 and user_mb < lim:
+        lim = user_mb
+    return lim
+
+
+def scene_bytes(receptor, scene) -> int:
+    """The scene's device memory (predict_gpu_fix, screening.cpp:44-51): receptor atoms, grid maps
+    and their cell-major copy."""
+    n, _, _, nt = scene.grid_info()
+    n0, n1, n2 = (int(v) for v in n)
+    maps = nt * n0 * n1 * n2 * 4 + (nt * (n0 - 1) * (n1 - 1) * (n2 - 1) * 32 if nt else 0)
+    return receptor.natom * 16 + maps
+
+
+def memory_batches(ligands, group, nchain: int, limit_mb: int):
+    """Batches of one size group: at most 1,000 ligands each, with the estimated device memory
+    under the budget (run_screening, screening.cpp:168-196)."""
+    done = 0
+    while done < len(group):
+        batch, est = [], BatchSize()
+        while done + len(batch) < len(group):
+            i = group[done + len(batch)]
+            nxt = est.add(ligands[i])
+            if len(batch) >= MAX_BATCH or nxt.bytes(nchain) // MIB >= limit_mb:
+                break
+            batch.append(i)
+            est = nxt
+        if not batch:
+            raise RuntimeError(f"Not enough GPU Memory to process at least one ligand in a batch of group {group[done]}")
+        done += len(batch)
+        yield batch
+
+
 def fmt_float(v: float) -> str:
     """A float as the reference's rapidjson writer prints it: shortest round-trip digits of the
     double value, at most 3 decimals (extra digits dropped, not rounded), at least one."""
@@ -220,7 +284,8 @@ def run_native(p: dict, text: str, inter_mode: int = N.MODE_GRID) -> list:
 def core_pipeline(p: dict, box: Box, doc: dict, inter_mode: int = N.MODE_GRID) -> list:
     """core_pipeline, src/screening/core.cpp:100-223, over an engine-JSON document (receptor under
     "receptor", one member per ligand, optional "score"): bias check, search-mode and task presets,
-    size groups and batches of at most 1,000 ligands, one docking run and one output file per batch.
+    size groups and batches of at most 1,000 ligands within the device-memory budget, one docking
+    run and one output file per batch.
     ``p`` holds DockingPipeline's keyword arguments. Returns the files written; raises on failure.
     This is the Python-driven twin of ud2_run_json (DockingPipeline.host = "python"); both make
     the same kernel calls with the same arguments and write the same bytes."""
@@ -244,12 +309,15 @@ def core_pipeline(p: dict, box: Box, doc: dict, inter_mode: int = N.MODE_GRID) -
         return []
     eng = DockEngine(receptor, box, mode=inter_mode)
     eng.ensure_types(ligands)
+    fix_mb = (scene_bytes(receptor, eng.scene) if inter_mode == N.MODE_GRID else receptor.natom * 16) // MIB
+    limit_mb = memory_limit_mb(p["max_gpu_mem"])
+    if limit_mb < fix_mb:
+        raise RuntimeError(f"GPU memory ({limit_mb} MB) is not enough for the receptor ({fix_mb} MB)")
     files = []
     batch_id = 0
     opt_steps = p["opt_steps"]
     for group in group_ligands(ligands):
-        for start in range(0, len(group), MAX_BATCH):
-            idx = group[start:start + MAX_BATCH]
+        for idx in memory_batches(ligands, group, p["exhaustiveness"], limit_mb - fix_mb):
             batch = [ligands[i] for i in idx]
             batch_id += 1
             if opt_steps == -1:  # heuristic of the first batch persists (screening.cpp:206-208)
