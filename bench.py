@@ -432,12 +432,14 @@ def main():
         "mc_ms_per_launch": round(float(mc_ms.mean()), 3) if launches else None,
         "energy_evals_per_step": int(st[0] / steps),
         "evals_per_sec": round(st[0] / wall, 1),
+        "mc_steps_per_sec": round(len(ligands) * nchain * msteps * steps / wall, 1),
         "median_top_affinity": round(float(np.nanmedian(top)), 4),
     }
     if learned:
         cfg["plain"] = {
             "value": round(p_value, 4), "unit": "ligands/s", "nchain": args.nchain, "mc_steps": args.mc_steps,
             "ms_per_step": round(p_wall * 1000.0 / p_steps, 3),
+            "mc_steps_per_sec": round(len(ligands) * args.nchain * args.mc_steps * p_steps / p_wall, 1),
             "median_top_affinity": round(float(np.nanmedian(p_top)), 4),
         }
         cfg["speedup_vs_plain"] = round(value / p_value, 3)
