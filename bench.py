@@ -330,7 +330,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(ligands are independent: see DESIGN.md for how the batch would shard)"}))
+                          "(ligands are independent: a multi-GPU run would shard the size-sorted ligand queue "
+                          "by estimated cost, one process per GPU, with no collective on the search path)"}))
         sys.exit(2)
     if args.workload == "score":
         return score_microbench(args)

@@ -273,8 +273,8 @@ def test_final_score_matches_oracle(gpu, scene, oracle_scenes, lig_c2, box):
 
 
 def test_mc_distribution_matches_oracle(gpu, scene, oracle_scenes, lig_c0):
-    """Plain MC (randomise + 8 steps x BFGS(8)) in grid mode, judged as the north star
-    asks: the distribution of the best score over 32 seeds (16 chains each) must match
+    """Plain MC (randomise + 8 steps x BFGS(8)) in grid mode, judged distributionally:
+    the distribution of the best score over 32 seeds (16 chains each) must match
     the FP64 oracle's (two-sample KS p > 0.01, medians within 0.25 kcal/mol). Chains
     use the same Philox streams as the oracle; individual trajectories may still split
     where float and double round a decision differently."""
