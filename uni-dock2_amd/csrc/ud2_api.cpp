@@ -114,6 +114,7 @@ struct ud2_ligset {
     float2* d_ranges = nullptr;
     unsigned long long* d_mask = nullptr;
     uint2* d_pairs = nullptr;
+    unsigned long long* d_fold = nullptr;
     int32_t* d_counter = nullptr;
     int32_t* d_order = nullptr;   // work order, n_lig entries
     int32_t* d_sel_all = nullptr; // scratch for all-pose selections
@@ -135,6 +136,7 @@ struct ud2_ligset {
         v.ranges = d_ranges;
         v.tors_mask = d_mask;
         v.pairs = d_pairs;
+        v.fold_mask = d_fold;
         v.bias_start = d_bias_start;
         v.bias_site = d_bias_site;
         v.bias_r2 = d_bias_r2;
@@ -483,6 +485,7 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
     std::vector<float2> ranges;
     std::vector<unsigned long long> mask(size_t(nt_all) * kMaxApl, 0ull);
     std::vector<uint2> pairs;
+    std::vector<unsigned long long> fold;
     std::vector<uint8_t> has_type(kNumTypes, 0);
     for (int l = 0; l < n; l++) {
         const int a0 = b->atom_off_h[l], na = b->atom_off_h[l + 1] - a0;
@@ -590,6 +593,28 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
             pairs.push_back(rec);
         }
         d.npair = int32_t(pairs.size()) - d.pair_off;
+        // The same pairs as folded diagonals for one-slot ligands: on diagonal d lane l scores
+        // {l, (l + d) mod na}; diagonals 1 .. na / 2 visit every unordered pair once (on the
+        // last diagonal of an even na only lanes below na / 2).
+        d.fold_off = int32_t(fold.size());
+        d.nfold = 0;
+        if (na <= kWave) {
+            d.nfold = na / 2;
+            std::vector<uint8_t> scored(size_t(na) * na, 0);
+            for (auto& ij : pl) scored[size_t(ij.first) * na + ij.second] = scored[size_t(ij.second) * na + ij.first] = 1;
+            size_t seen = 0;
+            for (int dd = 1; dd <= d.nfold; dd++) {
+                unsigned long long m = 0;
+                const int lanes = (2 * dd == na) ? na / 2 : na;
+                for (int l = 0; l < lanes; l++)
+                    if (scored[size_t(l) * na + (l + dd) % na]) {
+                        m |= 1ull << l;
+                        seen++;
+                    }
+                fold.push_back(m);
+            }
+            if (seen != pl.size()) return fail(UD2_ERR_STATE, tag + "pair diagonals do not cover the pairs");
+        }
     }
     ud2_ligset* ls = new ud2_ligset();
     hipGetDevice(&ls->device);
@@ -605,6 +630,7 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
     if (!rc) rc = upload(&ls->d_ranges, ranges);
     if (!rc) rc = upload(&ls->d_mask, mask);
     if (!rc) rc = upload(&ls->d_pairs, pairs);
+    if (!rc) rc = upload(&ls->d_fold, fold);
     if (!rc) rc = upload(&ls->d_counter, std::vector<int32_t>(kMaxApl + 1, 0));
     if (!rc) rc = upload(&ls->d_order, std::vector<int32_t>(n, 0));
     if (rc) {
@@ -618,7 +644,7 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
 int ud2_ligset_destroy(ud2_ligset* ls) {
     if (!ls) return UD2_OK;
     void* ptrs[] = {ls->d_ligs,   ls->d_x0,      ls->d_axis,    ls->d_dihe0,       ls->d_trange,
-                    ls->d_ranges, ls->d_mask,    ls->d_pairs,   ls->d_counter,     ls->d_order,
+                    ls->d_ranges, ls->d_mask,    ls->d_pairs,   ls->d_fold,        ls->d_counter,  ls->d_order,
                     ls->d_sel_all, ls->d_bias_start, ls->d_bias_site, ls->d_bias_r2};
     for (void* p : ptrs)
         if (p) hipFree(p);

@@ -272,6 +272,7 @@ struct Ctx {
     int natom, ntors, npair, nheavy, dim;
     int atom_off, tors_off, pair_off;
     int leaf_first;     // the torsion table's order is leaf-first (see apply_update)
+    int nfold, fold_off;  // folded pair diagonals (one-slot ligands)
     int type[APL];      // XS type; -1 for lanes beyond natom
     int slot[APL];      // grid map of the atom's type
     int lx, lf, lo, lt, lh, ld;
@@ -289,6 +290,7 @@ __device__ __forceinline__ void load_ctx(Ctx<APL>& C, const LigSetDev& L, const 
     C.dim = 6 + d.ntors;
     C.atom_off = d.atom_off; C.tors_off = d.tors_off; C.pair_off = d.pair_off;
     C.leaf_first = d.leaf_first;
+    C.nfold = d.nfold; C.fold_off = d.fold_off;
     C.L = &L; C.S = &S; C.mode = mode;
     C.evals = 0;
     C.lx = base;
@@ -665,7 +667,7 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
 #pragma unroll
     for (int k = 0; k < APL; k++) {
         const int a = ln + kWave * k;
-        reinterpret_cast<float4*>(s_lds + C.lx)[a] = make_float4(P.x[k], P.y[k], P.z[k], 0.0f);
+        reinterpret_cast<float4*>(s_lds + C.lx)[a] = make_float4(P.x[k], P.y[k], P.z[k], __int_as_float(C.type[k]));
         s_lds[C.lf + a] = 0.0f;
         s_lds[C.lf + n3 + a] = 0.0f;
         s_lds[C.lf + 2 * n3 + a] = 0.0f;
@@ -715,6 +717,50 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
         fx[k] = gx; fy[k] = gy; fz[k] = gz;
     }
     wave_lds_fence();
+    if constexpr (APL == 1) {
+        // intra-molecular pairs with the per-pair curl, each pair once, as folded diagonals: on
+        // diagonal d lane l scores the pair {l, (l + d) mod natom}; lane l keeps the pair force on
+        // its own atom in registers and hands the partner's share on by one lane rotation (the
+        // lane whose partner is l is (l - d) mod natom), so no LDS atomics are needed
+        const int n = C.natom;
+        const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
+        const unsigned long long* fm = C.L->fold_mask + C.fold_off;
+        const int ti = C.type[0];
+        const float ri = c_vdw[ti < 0 ? 0 : ti];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll 1
+        for (int dg = 1; dg <= C.nfold; dg++) {
+            const unsigned long long m = fm[dg - 1];
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+            if ((m >> ln) & 1ull) {
+                const int q = ln + dg < n ? ln + dg : ln + dg - n;
+                const float4 xj = lx4[q];
+                const float rx = xj.x - P.x[0], ry = xj.y - P.y[0], rz = xj.z - P.z[0];
+                const float rr = rx * rx + ry * ry + rz * rz;
+                if (rr < kR2Cut) {
+                    const int tj = __float_as_int(xj.w);
+                    float d = fsqrt(rr), dd;
+                    float ep = eval_ef(d - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
+                    d = fmaxf(d, kEps);
+                    const float fdr = dd * frcp(d);
+                    px = -fdr * rx; py = -fdr * ry; pz = -fdr * rz;
+                    // per-pair curl, branch-free
+                    float cs = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
+                    ep *= cs;
+                    cs *= cs;
+                    px *= cs; py *= cs; pz *= cs;
+                    e += ep;
+                }
+            }
+            ax += px; ay += py; az += pz;
+            const int src = ln - dg >= 0 ? ln - dg : ln - dg + n;
+            const float bx = __shfl(px, src), by = __shfl(py, src), bz = __shfl(pz, src);
+            if (ln < n) { ax -= bx; ay -= by; az -= bz; }
+        }
+        fx[0] += ax; fy[0] += ay; fz[0] += az;
+        if (C.L->bias_type != 0 && C.L->bias_start) e += bias_forces(C, P, F);
+        return uni(wave_sum(e));
+    }
     // intra-molecular pairs with the per-pair curl; each pair once
     const uint2* pr = C.L->pairs + C.pair_off;
     const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
