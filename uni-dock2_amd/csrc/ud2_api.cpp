@@ -1257,12 +1257,20 @@ int ud2_lmc(const ud2_scene* sc, const ud2_ligset* ls, const ud2_poses* P, const
         // the hidden layer (16 x 264 bf16) the Hessian's. The torsion table
         // between them stays in use.
         const int head = lds_head_floats(apl, A.hdim_ld);
-        const int a1_floats = kGroup * (kHidden + 8) / 2;
+        // hidden-layer rows: those that fit below the accepted pose's parking slot (coordinate and
+        // force tiles and the BFGS-start slot, idle while the network runs; the first rows overlay
+        // the candidate rows, which the network reads into registers first) go at the slice start,
+        // the others after the head
+        const int row_floats = kA1Stride / 2;
+        const int idle_floats = 7 * kWave * apl + (kWave * (3 * apl + 1) + 8);
+        A.a1_split = std::min(kGroup, idle_floats / row_floats);
+        const int a1_floats = (kGroup - A.a1_split) * row_floats;
         const int shared = feats ? std::max(dmax * A.hdim_ld, std::max(8 * kWave * apl, a1_floats)) : dmax * A.hdim_ld;
         static_assert(kGroup * kFeat / 2 <= 7 * kWave + kWave * 4 + 8, "candidate rows exceed the idle LDS");
         A.lfeat = 0;
         A.lscr = head;
-        A.la1 = head;
+        A.la1 = 0;
+        A.la1b = head;
         A.lds_floats_per_wave = lds_round4(head + shared);
         const int lds_bytes = A.lds_floats_per_wave * kWavesPerBlock * int(sizeof(float));
         if (lds_bytes > 160 * 1024) return fail(UD2_ERR_LIMIT, "per-block LDS exceeds 160 KB");

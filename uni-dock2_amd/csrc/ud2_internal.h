@@ -170,6 +170,7 @@ constexpr int kFeat = 80;     // network inputs of one candidate
 constexpr int kPooled = 64;   // 8 feature classes x 8 statistics
 constexpr int kHidden = 256;
 constexpr int kGroup = 16;    // candidates of one MC state = MFMA tile width
+constexpr int kA1Stride = kHidden + 8;  // bf16 per staged hidden row; padding spreads rows over the LDS banks
 constexpr int kMaxCand = 16;
 constexpr int kMlpNParam = kFeat * kHidden + kHidden + kHidden * kHidden + kHidden + kHidden + 1;
 constexpr int kMlpNNorm = 2 * kFeat;
@@ -220,7 +221,8 @@ struct LmcArgs {
     int32_t lig_index_base;
     int32_t lds_floats_per_wave;
     int32_t hdim_ld;
-    int32_t lscr, lfeat, la1;  // feature scratch, candidate rows, hidden layer (floats from the slice start)
+    int32_t lscr, lfeat;       // feature scratch, candidate rows (floats from the slice start)
+    int32_t la1, la1b, a1_split;  // hidden layer: rows below a1_split at la1, the others at la1b
     int32_t policy, n_cand, top_k;
     float sel_temp;          // learned: > 0 draws one candidate with p ~ exp(-score / sel_temp)
     int32_t n_start;         // > 1: randomised starts per chain, the best kept (network or exact energy)

@@ -96,11 +96,13 @@ __device__ __forceinline__ void pose_features(const Ctx<APL>& C, const WPose<APL
     wave_lds_fence();
 }
 
-// Persistent learned-proposal MC kernel. One-atom-per-lane ligands fit three
-// waves per SIMD in LDS; the occupancy hint holds the register allocation to
-// the 168 VGPRs that allows.
+// Persistent learned-proposal MC kernel. One-atom-per-lane ligands fit four
+// waves per SIMD in LDS (the network's hidden rows are staged partly in the
+// slice's idle coordinate / force / BFGS-start area); the occupancy hint holds
+// the register allocation to the 128 VGPRs that allows, which measured faster
+// than three waves without spills.
 template <int APL>
-__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(APL == 1 ? 3 : 1)))
+__global__ __launch_bounds__(kWave* kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(APL == 1 ? 4 : 1)))
 void lmc_kernel(LmcArgs A) {
     const int w = threadIdx.x / kWave;
     const int base = w * A.lds_floats_per_wave;
@@ -109,6 +111,7 @@ void lmc_kernel(LmcArgs A) {
     const int ln = lane();
     uint16_t* xl = reinterpret_cast<uint16_t*>(s_lds + base + A.lfeat);
     uint16_t* a1l = reinterpret_cast<uint16_t*>(s_lds + base + A.la1);
+    uint16_t* a1lb = reinterpret_cast<uint16_t*>(s_lds + base + A.la1b);
     for (;;) {
         int item = 0;
         if (ln == 0) item = atomicAdd(A.counter, 1);
@@ -199,7 +202,7 @@ void lmc_kernel(LmcArgs A) {
                 n_feat += nc;
                 if (A.policy == kPolicyLearned) {
                     wave_lds_fence();
-                    float p = mlp_group_scores_lds(A.W, xl, a1l);  // lane 16 q + r: score of candidate r
+                    float p = mlp_group_scores_lds(A.W, xl, a1l, a1lb, A.a1_split);  // lane 16 q + r: score of candidate r
                     p = (ln < nc && p == p) ? p : INFINITY;
                     int rank = 0;
                     float pmin = INFINITY;

@@ -114,14 +114,16 @@ __device__ __forceinline__ float mlp_group_scores(const MlpDev& W, const uint16_
 
 // The same scores with the hidden activations staged through LDS instead of
 // registers (for kernels whose other phases need the registers): x holds the
-// 16 bf16 input rows, a1 receives [16][kA1Stride] bf16 (both LDS, 8-byte
-// aligned, private to the wave).
-constexpr int kA1Stride = kHidden + 8;  // padded rows spread the 16 candidates over the LDS banks
+// 16 bf16 input rows, the hidden layer is [16][kA1Stride] bf16 with rows below
+// `split` at a1 and the others at a1b (all LDS, 8-byte aligned, private to the
+// wave). The rows at a1 may overlay x: x is read into registers first.
 
-__device__ __forceinline__ float mlp_group_scores_lds(const MlpDev& W, const uint16_t* x, uint16_t* a1) {
+__device__ __forceinline__ float mlp_group_scores_lds(const MlpDev& W, const uint16_t* x, uint16_t* a1,
+                                                      uint16_t* a1b, int split) {
     const int l = threadIdx.x % kWave, row = l & 15, q = l >> 4;
     bf16x4 xb[5];
     load_inputs(W, x + row * kFeat, q, xb);
+    uint16_t* a1row = row < split ? a1 + row * kA1Stride : a1b + (row - split) * kA1Stride;
     for (int mt = 0; mt < 16; mt++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         const uint16_t* wr = W.w1t + (16 * mt + row) * kFeat + 4 * q;
@@ -130,7 +132,7 @@ __device__ __forceinline__ float mlp_group_scores_lds(const MlpDev& W, const uin
         bf16x4 h;
 #pragma unroll
         for (int i = 0; i < 4; i++) h[i] = short(f2bf(fmaxf(acc[i] + W.b1[16 * mt + 4 * q + i], 0.0f)));
-        *reinterpret_cast<bf16x4*>(a1 + row * kA1Stride + 16 * mt + 4 * q) = h;
+        *reinterpret_cast<bf16x4*>(a1row + 16 * mt + 4 * q) = h;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -138,7 +140,7 @@ __device__ __forceinline__ float mlp_group_scores_lds(const MlpDev& W, const uin
     for (int ot = 0; ot < 16; ot++) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         const uint16_t* wr = W.w2t + (16 * ot + row) * kHidden + 4 * q;
-        const uint16_t* ar = a1 + row * kA1Stride + 4 * q;
+        const uint16_t* ar = a1row + 4 * q;
 #pragma unroll 4
         for (int kt = 0; kt < 16; kt++) acc = mfma16(ld4(wr + 16 * kt), ld4(ar + 16 * kt), acc);
 #pragma unroll
