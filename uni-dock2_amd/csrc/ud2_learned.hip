@@ -73,6 +73,7 @@ __device__ __forceinline__ void pose_features(const Ctx<APL>& C, const WPose<APL
     const float my_cls = float(ln >> 3);
     const int st = ln & 7;
     float f = 0.0f;
+#pragma unroll 4
     for (int a = 0; a < C.natom; a++) {
         const float c = s_lds[sc + 8 * a];
         const float v = st == 0 ? 1.0f : s_lds[sc + 8 * a + st];
