@@ -185,27 +185,37 @@ class DockEngine:
         results = [LigandResult(l.name) for l in ligs]
         if not sync:
             return results, pb
-        sc = out4[:len(keep_ids)].cpu().numpy()
+        n_keep = len(keep_ids)
+        sc = out4[:n_keep].cpu().numpy().astype(np.float32)
         dc = decomp.cpu().numpy() if decomp is not None else None
         x_h = pb.x.cpu().numpy()
-        d_h = pb.dihe.cpu().numpy()
-        intra_r1 = {}
-        for k, (pid, l) in enumerate(zip(keep_ids, owner)):
+        d_deg = (pb.dihe.cpu().numpy() * RAD2DEG_F32).astype(np.float32)
+        # final energies of all kept poses at once, in FP32 in the order of run_score
+        # (DockTask.cpp:248-262): total = intra + inter + penalty; inter-molecular part against the
+        # ligand's rank-1 intra; affinity = that / (1 + w * ntors)
+        own = np.asarray(owner, np.int64)
+        intra, inter, pen, bias = sc[:, 0], sc[:, 1], sc[:, 2], sc[:, 3]
+        first = np.ones(n_keep, bool)
+        first[1:] = own[1:] != own[:-1]
+        intra_r1 = intra[np.maximum.accumulate(np.where(first, np.arange(n_keep), 0))]
+        total = (intra + inter) + pen
+        e_inter = total - intra_r1
+        ntors = np.asarray([ligs[l].ntors for l in owner], np.float32)
+        aff = e_inter / (np.float32(1) + W_CONF_INDEP * ntors)
+        energies = np.stack([aff, total, intra, inter, pen, aff - e_inter, bias], axis=1).astype(np.float32)
+        atom_off = ligset.arrays["atom_off"]
+        tors_off = ligset.arrays["tors_off"]
+        for k in range(n_keep):
+            pid, l = keep_ids[k], owner[k]
             L = ligs[l]
             ch = pid - l * nchain
-            intra, inter, pen, bias = (np.float32(v) for v in sc[k])
-            if l not in intra_r1:
-                intra_r1[l] = intra
-            total = np.float32(intra + inter + pen)
-            e_inter = np.float32(total - intra_r1[l])
-            aff = np.float32(e_inter / (np.float32(1) + W_CONF_INDEP * np.float32(L.ntors)))
             r = results[l]
-            xo = ligset.x_offset(l, nchain, ch)
-            do = ligset.dihe_offset(l, nchain, ch)
+            xo = (int(atom_off[l]) * nchain + ch * L.natom) * 3
+            do = int(tors_off[l]) * nchain + ch * L.ntors
             r.pose_ids.append(pid)
             r.coords.append(x_h[xo:xo + 3 * L.natom].copy())
-            r.dihedrals.append((d_h[do:do + L.ntors] * RAD2DEG_F32).astype(np.float32))
-            r.energies.append(np.asarray([aff, total, intra, inter, pen, np.float32(aff - e_inter), bias], np.float32))
+            r.dihedrals.append(d_deg[do:do + L.ntors].copy())
+            r.energies.append(energies[k])
             if dc is not None:
                 r.decomp.append(dc[k, :L.natom].copy())
         return results, pb
