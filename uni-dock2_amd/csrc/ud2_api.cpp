@@ -596,22 +596,23 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
         // The same pairs as folded diagonals for one-slot ligands: on diagonal d lane l scores
         // {l, (l + d) mod na}; diagonals 1 .. na / 2 visit every unordered pair once (on the
         // last diagonal of an even na only lanes below na / 2).
+        // Per diagonal, one 64-bit word per atom slot (atom a in bit a % 64 of word a / 64).
         d.fold_off = int32_t(fold.size());
-        d.nfold = 0;
-        if (na <= kWave) {
-            d.nfold = na / 2;
+        d.nfold = na / 2;
+        {
+            const int apl = (na + kWave - 1) / kWave;
             std::vector<uint8_t> scored(size_t(na) * na, 0);
             for (auto& ij : pl) scored[size_t(ij.first) * na + ij.second] = scored[size_t(ij.second) * na + ij.first] = 1;
             size_t seen = 0;
             for (int dd = 1; dd <= d.nfold; dd++) {
-                unsigned long long m = 0;
+                std::vector<unsigned long long> m(apl, 0ull);
                 const int lanes = (2 * dd == na) ? na / 2 : na;
                 for (int l = 0; l < lanes; l++)
                     if (scored[size_t(l) * na + (l + dd) % na]) {
-                        m |= 1ull << l;
+                        m[l / kWave] |= 1ull << (l % kWave);
                         seen++;
                     }
-                fold.push_back(m);
+                fold.insert(fold.end(), m.begin(), m.end());
             }
             if (seen != pl.size()) return fail(UD2_ERR_STATE, tag + "pair diagonals do not cover the pairs");
         }

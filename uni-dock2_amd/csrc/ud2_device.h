@@ -65,6 +65,13 @@ __device__ __forceinline__ float wave_sum(float v) {
     return ((rdl(v, 0) + rdl(v, 16)) + rdl(v, 32)) + rdl(v, 48);
 }
 
+// Squared distance with explicit fused multiply-adds, so that every kernel that inlines it rounds it
+// the same way: the 8 A cutoff test on it must not depend on which kernel evaluates a pose (a pair
+// entering or leaving the cutoff moves the energy by up to about 0.004).
+__device__ __forceinline__ float sq_dist(float rx, float ry, float rz) {
+    return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+}
+
 // Hardware reciprocal / square root / reciprocal square root (about 1 ulp).
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -687,7 +694,7 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
             for (int j = 0; j < S.nrec; j++) {
                 const float4 r = S.rec[j];
                 const float rx = r.x - P.x[k], ry = r.y - P.y[k], rz = r.z - P.z[k];
-                const float rr = rx * rx + ry * ry + rz * rz;
+                const float rr = sq_dist(rx, ry, rz);
                 if (rr < kR2Cut) {
                     const int tj = __float_as_int(r.w);
                     float d = fsqrt(rr), dd;
@@ -736,7 +743,7 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
                 const int q = ln + dg < n ? ln + dg : ln + dg - n;
                 const float4 xj = lx4[q];
                 const float rx = xj.x - P.x[0], ry = xj.y - P.y[0], rz = xj.z - P.z[0];
-                const float rr = rx * rx + ry * ry + rz * rz;
+                const float rr = sq_dist(rx, ry, rz);
                 if (rr < kR2Cut) {
                     const int tj = __float_as_int(xj.w);
                     float d = fsqrt(rr), dd;
@@ -761,47 +768,66 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
         if (C.L->bias_type != 0 && C.L->bias_start) e += bias_forces(C, P, F);
         return uni(wave_sum(e));
     }
-    // intra-molecular pairs with the per-pair curl; each pair once
-    const uint2* pr = C.L->pairs + C.pair_off;
-    const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
-    // the next pair record is loaded while the current one is scored
-    uint2 rec_next = ln < C.npair ? pr[ln] : make_uint2(0u, 0u);
-#pragma unroll 1
-    for (int p = ln; p < C.npair; p += kWave) {
-        const uint2 rec = rec_next;
-        if (p + kWave < C.npair) rec_next = pr[p + kWave];
-        const int i = rec.x & 255, j = (rec.x >> 8) & 255, fl = (rec.x >> 16) & 3;
-        const float rs = __uint_as_float(rec.y);
-        const float4 xi = lx4[i], xj = lx4[j];
-        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-        const float rr = rx * rx + ry * ry + rz * rz;
-        if (rr < kR2Cut) {
-            float d = fsqrt(rr), dd;
-            float ep = eval_ef(d - rs, fl, &dd);
-            d = fmaxf(d, kEps);
-            const float fdr = dd * frcp(d);
-            float px = -fdr * rx, py = -fdr * ry, pz = -fdr * rz;
-            // per-pair curl, branch-free
-            float s = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
-            ep *= s;
-            s *= s;
-            px *= s; py *= s; pz *= s;
-            e += ep;
-            atomicAdd(&s_lds[C.lf + i], px);
-            atomicAdd(&s_lds[C.lf + n3 + i], py);
-            atomicAdd(&s_lds[C.lf + 2 * n3 + i], pz);
-            atomicAdd(&s_lds[C.lf + j], -px);
-            atomicAdd(&s_lds[C.lf + n3 + j], -py);
-            atomicAdd(&s_lds[C.lf + 2 * n3 + j], -pz);
-        }
-    }
-    wave_lds_fence();
+    // intra-molecular pairs with the per-pair curl, each pair once, as folded diagonals over the
+    // atom slots: on diagonal d atom a scores the pair {a, (a + d) mod natom} and keeps the pair
+    // force on itself; the partner's share is staged in the pair-force tile (one plain write per
+    // atom) and read back by the atom (c - d) mod natom names, so no LDS atomics are needed
+    {
+        const int n = C.natom;
+        const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
+        const unsigned long long* fm = C.L->fold_mask + C.fold_off;
+        float ax[APL], ay[APL], az[APL];
 #pragma unroll
-    for (int k = 0; k < APL; k++) {
-        const int a = ln + kWave * k;
-        fx[k] += s_lds[C.lf + a];
-        fy[k] += s_lds[C.lf + n3 + a];
-        fz[k] += s_lds[C.lf + 2 * n3 + a];
+        for (int k = 0; k < APL; k++) ax[k] = ay[k] = az[k] = 0.0f;
+#pragma unroll 1
+        for (int dg = 1; dg <= C.nfold; dg++) {
+#pragma unroll
+            for (int k = 0; k < APL; k++) {
+                const unsigned long long m = fm[(dg - 1) * APL + k];
+                const int a = ln + kWave * k;
+                float px = 0.0f, py = 0.0f, pz = 0.0f;
+                if ((m >> ln) & 1ull) {
+                    const int ti = C.type[k];
+                    const int q = a + dg < n ? a + dg : a + dg - n;
+                    const float4 xj = lx4[q];
+                    const float rx = xj.x - P.x[k], ry = xj.y - P.y[k], rz = xj.z - P.z[k];
+                    const float rr = sq_dist(rx, ry, rz);
+                    if (rr < kR2Cut) {
+                        const int tj = __float_as_int(xj.w);
+                        float d = fsqrt(rr), dd;
+                        float ep = eval_ef(d - (c_vdw[ti] + c_vdw[tj]), pair_flags(ti, tj), &dd);
+                        d = fmaxf(d, kEps);
+                        const float fdr = dd * frcp(d);
+                        px = -fdr * rx; py = -fdr * ry; pz = -fdr * rz;
+                        float cs = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
+                        ep *= cs;
+                        cs *= cs;
+                        px *= cs; py *= cs; pz *= cs;
+                        e += ep;
+                    }
+                }
+                ax[k] += px; ay[k] += py; az[k] += pz;
+                s_lds[C.lf + a] = px;
+                s_lds[C.lf + n3 + a] = py;
+                s_lds[C.lf + 2 * n3 + a] = pz;
+            }
+            wave_lds_fence();
+#pragma unroll
+            for (int k = 0; k < APL; k++) {
+                const int c = ln + kWave * k;
+                if (c < n) {
+                    const int src = c - dg >= 0 ? c - dg : c - dg + n;
+                    ax[k] -= s_lds[C.lf + src];
+                    ay[k] -= s_lds[C.lf + n3 + src];
+                    az[k] -= s_lds[C.lf + 2 * n3 + src];
+                }
+            }
+            wave_lds_fence();
+        }
+#pragma unroll
+        for (int k = 0; k < APL; k++) {
+            fx[k] += ax[k]; fy[k] += ay[k]; fz[k] += az[k];
+        }
     }
     if (C.L->bias_type != 0 && C.L->bias_start) e += bias_forces(C, P, F);
     return uni(wave_sum(e));

@@ -23,7 +23,7 @@ struct DevLig {
     int32_t apl;        // ceil(natom / 64)
     int32_t leaf_first; // 1: the torsions have a leaf-first application order (tors_axis .y >> 16)
     int32_t fold_off;   // into fold_mask
-    int32_t nfold;      // natom / 2 folded pair diagonals when natom <= 64, else 0
+    int32_t nfold;      // natom / 2 folded pair diagonals, apl mask words each
 };
 
 // Device view of a ligand set.
@@ -36,9 +36,10 @@ struct LigSetDev {
     const float2* ranges;                  // (lo, hi) radians
     const unsigned long long* tors_mask;   // [ntors_total][kMaxApl] rotated-atom bit masks
     const uint2* pairs;                    // {i | j << 8 | hyd << 16 | hbond << 17, bits of R_i + R_j}
-    // Folded pair diagonals of ligands with at most 64 atoms: bit l of mask d - 1 is set when the
-    // pair {l, (l + d) mod natom} is a scored (heavy) intra pair, for d = 1 .. natom / 2 (for even
-    // natom, diagonal natom / 2 only on lanes below natom / 2, so every pair appears once).
+    // Folded pair diagonals: for d = 1 .. natom / 2, apl words per diagonal; bit a % 64 of word
+    // (d - 1) * apl + a / 64 is set when the pair {a, (a + d) mod natom} is a scored (heavy) intra
+    // pair (for even natom, diagonal natom / 2 only for atoms below natom / 2, so every pair
+    // appears once).
     const unsigned long long* fold_mask;
     // Bias sites (FlexParamVina::inds_bias / params_bias, E/src/cuda/task.cu:157-168)
     // and the task options that change the search (DockParam, E/src/model/model.h:151-168).
