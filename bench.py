@@ -280,15 +280,21 @@ DEFAULT_TRAFFIC_CONFIG = "1024 ligands grid; learned 36x40 starts 4 cand 16 T 1.
 
 
 def kernel_sources_digest() -> str:
-    """SHA-1 (12 hex digits) of the sources that define and launch the kernels (the device code and
-    ud2_api.cpp; not the whole-run orchestration): a traffic summary counts only for the build it measured."""
+    """SHA-1 (12 hex digits) of the code that defines and launches the kernels (the device sources and
+    ud2_api.cpp, comments and blank lines left out; not the whole-run orchestration): a traffic summary
+    counts only for the build it measured."""
     import hashlib
+    import re
     d = os.path.join(ROOT, "uni-dock2_amd", "csrc")
     h = hashlib.sha1()
     for f in sorted(os.listdir(d)):
         if f.endswith((".h", ".hip")) or f == "ud2_api.cpp":
+            text = open(os.path.join(d, f), encoding="utf-8").read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            text = re.sub(r"//[^\n]*", "", text)
+            code = "\n".join(line.rstrip() for line in text.splitlines() if line.strip())
             h.update(f.encode())
-            h.update(open(os.path.join(d, f), "rb").read())
+            h.update(code.encode())
     return h.hexdigest()[:12]
 
 

@@ -567,9 +567,8 @@ int ud2_ligset_create(const ud2_ligand_batch* b, ud2_ligset** out) {
             for (int k = 0; k < nt; k++) axis[t0 + k].y |= ord[k] << 16;
             d.leaf_first = ok ? 1 : 0;
         }
-        // Heavy pairs in diagonal order (j - i, i): 64 consecutive pairs then hit
-        // distinct atoms on each side, so the kernel's LDS force atomics of one
-        // instruction do not collide.
+        // Heavy pairs as records for the final score (each pair once, i < j), in
+        // diagonal order (j - i, i).
         std::vector<std::pair<int, int>> pl;
         for (int k = 0; k < np; k++) {
             int i = b->pairs_h[2 * (p0 + k)], j = b->pairs_h[2 * (p0 + k) + 1];
