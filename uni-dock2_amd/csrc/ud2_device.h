@@ -7,9 +7,11 @@
 // the dihedral vector and every BFGS vector (gradient, direction, y, -Hy) are
 // one VGPR each with component k on lane k; the inverse Hessian is a per-wave
 // LDS tile whose row i only lane i reads and writes. Energies, torques and
-// dot products are wave reductions. Intra-molecular pairs are spread over the
-// lanes and scatter their forces into a per-wave LDS force tile with LDS
-// atomics. MFMA is not used: the physics is gather- and VALU-bound.
+// dot products are wave reductions. Intra-molecular pairs are scored along
+// folded diagonals (atom a with atom (a + d) mod natom), each atom keeping its
+// own pair forces in registers and passing the partner's share on by a lane
+// rotation (or, past 64 atoms, through a per-wave LDS tile), with no atomics.
+// MFMA is not used: the physics is gather- and VALU-bound.
 //
 // The algorithm follows the reference's device code (E/ =
 // unidock/unidock_engine/): cal_e_f_tile / cal_grad_tile (E/src/cuda/bfgs.cuh:
