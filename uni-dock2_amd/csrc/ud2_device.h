@@ -660,6 +660,28 @@ __device__ __forceinline__ float bias_forces(const Ctx<APL>& C, const WPose<APL>
     return e;
 }
 
+// One intra pair (bfgs.cuh:305-353): the force on atom xi (type ti) from atom xj (x, y, z, type
+// bits in w) under the per-pair curl with cap v_cap, added to e; zero beyond the 8 A cutoff.
+__device__ __forceinline__ void pair_force(float4 xi, int ti, float4 xj, float v_cap, float& px, float& py,
+                                           float& pz, float& e) {
+    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+    const float rr = sq_dist(rx, ry, rz);
+    if (rr < kR2Cut) {
+        const int tj = __float_as_int(xj.w);
+        float d = fsqrt(rr), dd;
+        float ep = eval_ef(d - (c_vdw[ti] + c_vdw[tj]), pair_flags(ti, tj), &dd);
+        d = fmaxf(d, kEps);
+        const float fdr = dd * frcp(d);
+        px = -fdr * rx; py = -fdr * ry; pz = -fdr * rz;
+        // per-pair curl, branch-free
+        float cs = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
+        ep *= cs;
+        cs *= cs;
+        px *= cs; py *= cs; pz *= cs;
+        e += ep;
+    }
+}
+
 // cal_e_f_tile (bfgs.cuh:221-394): energy under the curl cap v_cap and the
 // per-atom gradient F.
 template <int APL>
@@ -734,37 +756,51 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
         const int n = C.natom;
         const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
         const unsigned long long* fm = C.L->fold_mask + C.fold_off;
-        const int ti = C.type[0];
-        const float ri = c_vdw[ti < 0 ? 0 : ti];
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (n <= kWave / 2) {
+            // up to 32 atoms: two diagonals per step, lanes 0-31 on diagonal dg and lanes 32-63 on
+            // dg + 1, both for atom l % 32; lane a collects its atom's own share from lane a + 32
+            // and the partner shares of both diagonals
+            const int h = ln >> 5, a = ln & 31;
+            const float4 xa = lx4[a];
+            const int ta = __float_as_int(xa.w);
 #pragma unroll 1
-        for (int dg = 1; dg <= C.nfold; dg++) {
-            const unsigned long long m = fm[dg - 1];
-            float px = 0.0f, py = 0.0f, pz = 0.0f;
-            if ((m >> ln) & 1ull) {
-                const int q = ln + dg < n ? ln + dg : ln + dg - n;
-                const float4 xj = lx4[q];
-                const float rx = xj.x - P.x[0], ry = xj.y - P.y[0], rz = xj.z - P.z[0];
-                const float rr = sq_dist(rx, ry, rz);
-                if (rr < kR2Cut) {
-                    const int tj = __float_as_int(xj.w);
-                    float d = fsqrt(rr), dd;
-                    float ep = eval_ef(d - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
-                    d = fmaxf(d, kEps);
-                    const float fdr = dd * frcp(d);
-                    px = -fdr * rx; py = -fdr * ry; pz = -fdr * rz;
-                    // per-pair curl, branch-free
-                    float cs = ep > 0.0f ? v_cap * frcp(v_cap + ep) : 1.0f;
-                    ep *= cs;
-                    cs *= cs;
-                    px *= cs; py *= cs; pz *= cs;
-                    e += ep;
+            for (int dg = 1; dg <= C.nfold; dg += 2) {
+                const unsigned long long m0 = fm[dg - 1];
+                const unsigned long long m1 = dg < C.nfold ? fm[dg] : 0ull;
+                const unsigned long long m = h ? m1 : m0;
+                float px = 0.0f, py = 0.0f, pz = 0.0f;
+                if (a < n && ((m >> a) & 1ull)) {
+                    const int dd = dg + h;
+                    const int q = a + dd < n ? a + dd : a + dd - n;
+                    pair_force(xa, ta, lx4[q], v_cap, px, py, pz, e);
+                }
+                const float ox = __shfl(px, a + 32), oy = __shfl(py, a + 32), oz = __shfl(pz, a + 32);
+                const int s0 = a - dg >= 0 ? a - dg : a - dg + n;
+                const int s1 = 32 + (a - dg - 1 >= 0 ? a - dg - 1 : a - dg - 1 + n);
+                const float b0x = __shfl(px, s0), b0y = __shfl(py, s0), b0z = __shfl(pz, s0);
+                const float b1x = __shfl(px, s1), b1y = __shfl(py, s1), b1z = __shfl(pz, s1);
+                if (h == 0 && a < n) {
+                    ax += (px + ox) - (b0x + b1x);
+                    ay += (py + oy) - (b0y + b1y);
+                    az += (pz + oz) - (b0z + b1z);
                 }
             }
-            ax += px; ay += py; az += pz;
-            const int src = ln - dg >= 0 ? ln - dg : ln - dg + n;
-            const float bx = __shfl(px, src), by = __shfl(py, src), bz = __shfl(pz, src);
-            if (ln < n) { ax -= bx; ay -= by; az -= bz; }
+        } else {
+            const float4 xa = make_float4(P.x[0], P.y[0], P.z[0], __int_as_float(C.type[0]));
+#pragma unroll 1
+            for (int dg = 1; dg <= C.nfold; dg++) {
+                const unsigned long long m = fm[dg - 1];
+                float px = 0.0f, py = 0.0f, pz = 0.0f;
+                if ((m >> ln) & 1ull) {
+                    const int q = ln + dg < n ? ln + dg : ln + dg - n;
+                    pair_force(xa, C.type[0], lx4[q], v_cap, px, py, pz, e);
+                }
+                ax += px; ay += py; az += pz;
+                const int src = ln - dg >= 0 ? ln - dg : ln - dg + n;
+                const float bx = __shfl(px, src), by = __shfl(py, src), bz = __shfl(pz, src);
+                if (ln < n) { ax -= bx; ay -= by; az -= bz; }
+            }
         }
         fx[0] += ax; fy[0] += ay; fz[0] += az;
         if (C.L->bias_type != 0 && C.L->bias_start) e += bias_forces(C, P, F);
