@@ -809,7 +809,8 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
     // intra-molecular pairs with the per-pair curl, each pair once, as folded diagonals over the
     // atom slots: on diagonal d atom a scores the pair {a, (a + d) mod natom} and keeps the pair
     // force on itself; the partner's share is staged in the pair-force tile (one plain write per
-    // atom) and read back by the atom (c - d) mod natom names, so no LDS atomics are needed
+    // atom) and atom c reads it back from the slot of atom (c - d) mod natom, so no LDS atomics
+    // are needed
     {
         const int n = C.natom;
         const float4* lx4 = reinterpret_cast<const float4*>(s_lds + C.lx);
