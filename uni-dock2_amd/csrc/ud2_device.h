@@ -660,16 +660,16 @@ __device__ __forceinline__ float bias_forces(const Ctx<APL>& C, const WPose<APL>
     return e;
 }
 
-// One intra pair (bfgs.cuh:305-353): the force on atom xi (type ti) from atom xj (x, y, z, type
-// bits in w) under the per-pair curl with cap v_cap, added to e; zero beyond the 8 A cutoff.
-__device__ __forceinline__ void pair_force(float4 xi, int ti, float4 xj, float v_cap, float& px, float& py,
-                                           float& pz, float& e) {
+// One intra pair (bfgs.cuh:305-353): the force on atom xi (type ti, radius ri) from atom xj (x, y, z,
+// type bits in w) under the per-pair curl with cap v_cap, added to e; zero beyond the 8 A cutoff.
+__device__ __forceinline__ void pair_force(float4 xi, int ti, float ri, float4 xj, float v_cap, float& px,
+                                           float& py, float& pz, float& e) {
     const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
     const float rr = sq_dist(rx, ry, rz);
     if (rr < kR2Cut) {
         const int tj = __float_as_int(xj.w);
         float d = fsqrt(rr), dd;
-        float ep = eval_ef(d - (c_vdw[ti] + c_vdw[tj]), pair_flags(ti, tj), &dd);
+        float ep = eval_ef(d - (ri + c_vdw[tj]), pair_flags(ti, tj), &dd);
         d = fmaxf(d, kEps);
         const float fdr = dd * frcp(d);
         px = -fdr * rx; py = -fdr * ry; pz = -fdr * rz;
@@ -764,6 +764,7 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
             const int h = ln >> 5, a = ln & 31;
             const float4 xa = lx4[a];
             const int ta = __float_as_int(xa.w);
+            const float ra = c_vdw[ta < 0 ? 0 : ta];
 #pragma unroll 1
             for (int dg = 1; dg <= C.nfold; dg += 2) {
                 const unsigned long long m0 = fm[dg - 1];
@@ -773,7 +774,7 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
                 if (a < n && ((m >> a) & 1ull)) {
                     const int dd = dg + h;
                     const int q = a + dd < n ? a + dd : a + dd - n;
-                    pair_force(xa, ta, lx4[q], v_cap, px, py, pz, e);
+                    pair_force(xa, ta, ra, lx4[q], v_cap, px, py, pz, e);
                 }
                 const float ox = __shfl(px, a + 32), oy = __shfl(py, a + 32), oz = __shfl(pz, a + 32);
                 const int s0 = a - dg >= 0 ? a - dg : a - dg + n;
@@ -788,13 +789,14 @@ __device__ __forceinline__ float energy_forces(const Ctx<APL>& C, const WPose<AP
             }
         } else {
             const float4 xa = make_float4(P.x[0], P.y[0], P.z[0], __int_as_float(C.type[0]));
+            const float ra = c_vdw[C.type[0] < 0 ? 0 : C.type[0]];
 #pragma unroll 1
             for (int dg = 1; dg <= C.nfold; dg++) {
                 const unsigned long long m = fm[dg - 1];
                 float px = 0.0f, py = 0.0f, pz = 0.0f;
                 if ((m >> ln) & 1ull) {
                     const int q = ln + dg < n ? ln + dg : ln + dg - n;
-                    pair_force(xa, C.type[0], lx4[q], v_cap, px, py, pz, e);
+                    pair_force(xa, C.type[0], ra, lx4[q], v_cap, px, py, pz, e);
                 }
                 ax += px; ay += py; az += pz;
                 const int src = ln - dg >= 0 ? ln - dg : ln - dg + n;
